@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Throughput of the FP64 stencil suite on one MI355X.
+
+Workload "suite-fp64": one step = one launch of each of the twelve FP64
+stencils of the suite at the benchmark configuration (3D tests at
+512x256x256, 2D tests at 512x65536 -- the reference's `./benchmark 512 256
+256 ...` sizes), inputs already resident in HBM, ping-pong buffers swapped
+between steps as the reference's main() does.  `value` is grid points
+processed per second over the timed region (12 x 33,554,432 points per step).
+
+Per-kernel durations come from HIP events recorded around every launch on
+the launching stream (kg_timing_drain).  The roofline entry is for the kernel
+with the largest share of the step: algorithmic bytes per point (each array
+read once, each output written once) x points / mean launch duration, against
+8 TB/s.  The CPU baseline times the reference's own kernel functions
+(oracle/_ref/libref_fast.so: compiled from the reference sources with the
+reference's gcc flags) on one host core over a bounded sample.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "kernelgen-perf-tests_amd"
+HBM_PEAK_GBS = 8000.0
+METRIC = "grid-pts/s + HBM GB/s (% of 8 TB/s) per stencil; served p50/p99 latency"
+
+NX, NY, NS = 512, 256, 256
+N = NX * NY * NS
+DIMS3 = (NX, NY, NS)
+DIMS2 = (NX, NY * NS)
+
+# name -> (dims, algorithmic bytes per point, coefficient values, array roles,
+#          pairs rotated between steps).  Coefficients: the reference draws
+# them from rand(); fixed representative values are used here (timing does
+# not depend on them).
+STENCILS = {
+    "laplacian": (DIMS3, 16, (0.1, 0.2), ("w0", "w1"), [("w0", "w1")]),
+    "divergence": (DIMS3, 32, (0.5, -0.25, 0.75), ("u", "ux", "uy", "uz"), []),
+    "gradient": (DIMS3, 32, (0.5, -0.25, 0.75), ("u", "ux", "uy", "uz"), []),
+    "wave13pt": (DIMS3, 24, (0.6, 0.1, -0.05), ("w0", "w1", "w2"), [("w0", "w1", "w2")]),
+    "uxx1": (DIMS3, 48, (0.5, -0.3), ("u0", "u1", "d1", "xx", "xy", "xz"), [("u0", "u1")]),
+    "tricubic": (DIMS3, 40, (), ("u0", "u1", "a", "b", "c"), [("u0", "u1")]),
+    "tricubic2": (DIMS3, 40, (), ("u0", "u1", "a", "b", "c"), [("u0", "u1")]),
+    "lapgsrb": (DIMS3, 16, (0.5, 0.05, -0.02, 0.03), ("w0", "w1"), [("w0", "w1")]),
+    "jacobi": (DIMS2, 16, (0.5, 0.1, 0.02), ("w0", "w1"), [("w0", "w1")]),
+    "gaussblur": (DIMS2, 16, (0.4, 0.3, 0.2, 0.1, 0.05, 0.02), ("w0", "w1"), [("w0", "w1")]),
+    "whispering": (DIMS2, 96, (0.7, 0.3),
+                   ("e", "e_next", "h0", "h1", "u_em0", "u_em1", "ca", "cb", "da", "db"),
+                   [("e", "e_next"), ("h0", "h1"), ("u_em0", "u_em1")]),
+    "gameoflife": (DIMS2, 16, (), ("u0", "u1"), [("u0", "u1")]),
+}
+# Kernel symbol the C ABI reports for each entry point.
+KNAME = {"jacobi": "jacobi_"}
+SIZES = {"e": 2, "e_next": 2}
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("kgstencil", PKG / "kgstencil.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def rotate(bufs: dict, pairs) -> None:
+    for grp in pairs:
+        vals = [bufs[r] for r in grp]
+        for r, v in zip(grp, vals[1:] + vals[:1]):
+            bufs[r] = v
+
+
+def make_inputs(rng, roles, sizes):
+    out = {}
+    for r in roles:
+        n = N * sizes.get(r, 1)
+        a = rng.uniform(-1.0, 1.0, n)
+        if r == "d1":
+            a = np.abs(a) + 1.0          # density in [1,2): dth/d stays finite
+        out[r] = a
+    return out
+
+
+def gpu_leg(kg, steps: int, warmup: int, rng):
+    ctx = kg.Context(0)
+    bufs_by_stencil = {}
+    host_pool = {}
+    for name, (dims, bpp, coefs, roles, pairs) in STENCILS.items():
+        for r in roles:
+            key = (r, SIZES.get(r, 1))
+            if key not in host_pool:
+                host_pool[key] = make_inputs(rng, [r], SIZES)[r]
+        bufs_by_stencil[name] = {r: ctx.array(host_pool[(r, SIZES.get(r, 1))]) for r in roles}
+    if "whispering" in bufs_by_stencil:   # identical e borders in both e buffers
+        b = bufs_by_stencil["whispering"]
+        b["e_next"].copy_from(b["e"])
+    host_pool.clear()
+
+    def one_step():
+        for name, (dims, bpp, coefs, roles, pairs) in STENCILS.items():
+            b = bufs_by_stencil[name]
+            ctx.call(name, *dims, *coefs, *[b[r] for r in roles])
+            rotate(b, pairs)
+
+    for _ in range(warmup):
+        one_step()
+    ctx.sync()
+    ctx.drain()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    times = ctx.drain(cap=len(STENCILS) * steps + 16)
+    assert len(times) == len(STENCILS) * steps, (len(times), steps)
+    per = {}
+    names = list(STENCILS)
+    arr = np.array(times).reshape(steps, len(names))
+    for idx, name in enumerate(names):
+        dims, bpp, *_ = STENCILS[name]
+        t = float(arr[:, idx].mean())
+        per[name] = {
+            "us": round(t * 1e6, 2),
+            "gpts_s": round(N / t / 1e9, 2),
+            "gbs": round(bpp * N / t / 1e9, 1),
+            "frac_hbm": round(bpp * N / t / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_pt": bpp,
+            "regs": ctx.regcount(KNAME.get(name, name)),
+        }
+    ctx.close()
+    return wall, per
+
+
+def cpu_leg(reps: int, rng):
+    """Reference kernel functions (reference gcc flags) on one core, each
+    stencil `reps` times at the full configuration."""
+    lib_path = ROOT / "oracle" / "_ref" / "libref_fast.so"
+    kind = "reference"
+    if not lib_path.exists():
+        lib_path = ROOT / "oracle" / "liboracle.so"
+        kind = "port"
+    if not lib_path.exists():
+        return None
+    lib = C.CDLL(str(lib_path))
+    pool = {}
+    total_t = 0.0
+    per = {}
+    for name, (dims, bpp, coefs, roles, pairs) in STENCILS.items():
+        roles = [r for r in roles if r != "e_next"]
+        arrs = {}
+        for r in roles:
+            key = (r, SIZES.get(r, 1))
+            if key not in pool:
+                pool[key] = make_inputs(rng, [r], SIZES)[r]
+            arrs[r] = pool[key]
+        fn_name = name + ("_" if name == "jacobi" else "") if kind == "reference" else "oracle_" + name
+        fn = getattr(lib, fn_name)
+        fn.restype = None
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            if kind == "reference" and name == "jacobi":
+                ints = [C.c_int(d) for d in dims]
+                cs = [C.c_double(c) for c in coefs]
+                fn(*[C.byref(x) for x in ints], *[C.byref(x) for x in cs],
+                   *[C.c_void_p(arrs[r].ctypes.data) for r in roles])
+            else:
+                fn(*[C.c_int(d) for d in dims], *[C.c_double(c) for c in coefs],
+                   *[C.c_void_p(arrs[r].ctypes.data) for r in roles])
+            rotate(arrs, [p for p in pairs if "e_next" not in p])
+        dt = (time.perf_counter() - t0) / reps
+        per[name] = round(N / dt / 1e9, 4)
+        total_t += dt
+    pool.clear()
+    return {
+        "value": round(len(STENCILS) * N / total_t, 1),
+        "unit": "grid-pts/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"each of the {len(STENCILS)} stencils run {reps}x at 512x256x256 / 512x65536 "
+                  f"on one host core ({lib_path.name}); per-stencil Gpts/s: {json.dumps(per)}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; the stencil path runs on a single "
+              "device (multi-GPU z-slab sharding is described in DESIGN.md, not built)",
+              file=sys.stderr)
+        return 2
+    rng = np.random.default_rng(2024)
+    kg = load_pkg()
+    wall, per = gpu_leg(kg, args.steps, args.warmup, rng)
+    value = len(STENCILS) * N * args.steps / wall
+    dom = max(per, key=lambda k: per[k]["us"])
+    d = per[dom]
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get(dom)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "grid-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic uniform[-1,1) grids (uxx1 density in [1,2)), resident in HBM",
+        "config": {"workload": "suite-fp64: 12 FP64 stencils x 1 step, 3D 512x256x256, "
+                               "2D 512x65536",
+                   "grid_3d": list(DIMS3), "grid_2d": list(DIMS2), "stencils": list(STENCILS)},
+        "roofline": {"bound": "hbm", "kernel": dom,
+                     "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch": d["bytes_per_pt"] * N, "mean_launch_us": d["us"]},
+        "per_stencil": per,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_leg(args.cpu_reps, rng)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,64 @@
+// kg_internal.h -- shared pieces of libkgstencil.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/kg_stencil.h"
+
+struct kg_ctx {
+    int device;
+    hipStream_t stream;
+    int timing;                       // per-launch event pairs on/off
+    hipEvent_t* ev_start;             // ring of KG_TIMING_RING pairs
+    hipEvent_t* ev_stop;
+    long long launches;               // launches recorded so far
+    long long drained;                // launches already returned by kg_timing_drain
+    const char* last_kernel;          // name of the most recent launch
+};
+
+// Records the most recent error message (thread-local) and returns the code.
+int kg_set_error(int code, const char* fmt, ...);
+
+// Wrap a HIP call: on failure record the message and return KG_ERR_HIP.
+#define KG_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t kg_e_ = (call);                                                         \
+        if (kg_e_ != hipSuccess)                                                           \
+            return kg_set_error(KG_ERR_HIP, "%s failed: %s (%s:%d)", #call,                \
+                                hipGetErrorString(kg_e_), __FILE__, __LINE__);             \
+    } while (0)
+
+// Launch bookkeeping used by every entry point: kg_launch_begin() records the
+// start event of the next ring slot, kg_launch_end() checks the launch and
+// records the stop event.
+int kg_launch_begin(kg_ctx* ctx, const char* name);
+int kg_launch_end(kg_ctx* ctx);
+
+// Registry of entry-point name -> kernel symbol, for kg_kernel_regcount.
+struct kg_kernel_entry {
+    const char* name;
+    const void* func;
+};
+void kg_register_kernel(const char* name, const void* func);
+
+// Validation helpers shared by the entry points.
+int kg_check_ctx(kg_ctx* ctx);
+int kg_check_dims3(int nx, int ny, int ns);
+int kg_check_dims2(int nx, int ny);
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; each XCD
+// has its own L2.  kg_xcd_swizzle maps the hardware workgroup id to a logical
+// tile id so that XCD x receives the contiguous logical range
+// [x*n/8, (x+1)*n/8): neighbouring tiles (which share halo cache lines) then
+// sit in the same L2.  Ids past the last multiple of 8 map to themselves.
+__device__ __forceinline__ unsigned kg_xcd_swizzle(unsigned b, unsigned n)
+{
+    const unsigned per = n >> 3;
+    if (b < (per << 3)) return (b & 7u) * per + (b >> 3);
+    return b;
+}
+
+#define KG_NUM_XCD 8
