@@ -4,7 +4,8 @@
 //
 // One wave64 owns a 16x16 tile of C and walks k in steps of 4.  Per step the
 // lane l supplies A(i0 + l%16, k + l/16) and B(k + l/16, j0 + l%16); the four
-// accumulators of lane l are C(i0 + 4*(l/16) + r, j0 + l%16), r = 0..3.  k
+// accumulators of lane l are C(i0 + l/16 + 4*r, j0 + l%16), r = 0..3 (the
+// accumulator layout of this instruction, checked on the device).  k
 // tails that are not a multiple of 4 are padded with zero products.  Sums are
 // formed by the matrix core in blocks of 4 fused products, so the result
 // differs from a strictly sequential k loop by rounding.
@@ -33,7 +34,7 @@ void matmul_k(int nx, int ny, int ns, int ntile_i, int ntile_j,
     v4d acc;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int i = i0 + 4 * lk + r, j = j0 + lr;
+        const int i = i0 + lk + 4 * r, j = j0 + lr;
         acc[r] = (i < nx && j < ns) ? C[(long long)i + (long long)nx * j] : 0.0;
     }
     const int ia = i0 + lr;
@@ -46,7 +47,7 @@ void matmul_k(int nx, int ny, int ns, int ntile_i, int ntile_j,
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-        const int i = i0 + 4 * lk + r, j = j0 + lr;
+        const int i = i0 + lk + 4 * r, j = j0 + lr;
         if (i < nx && j < ns) C[(long long)i + (long long)nx * j] = acc[r];
     }
 }

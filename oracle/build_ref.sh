@@ -20,7 +20,7 @@
 # function can be called directly; nothing is copied or stubbed.  The Fortran
 # kernels (jacobi, matmul) are built with amdflang, gfortran being absent.
 #
-# -march defaults to x86-64-v3 (AVX2+FMA) rather than native so that the
+# -march defaults to x86-64-v4 (AVX-512) rather than native so that the
 # prebuilt files run on any current x86-64 host; set REF_MARCH=native to match
 # makefile.in exactly on the machine that runs them.
 set -euo pipefail
@@ -28,7 +28,7 @@ set -euo pipefail
 REF=${REF:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
 OUT="$HERE/_ref"
-MARCH=${REF_MARCH:-x86-64-v3}
+MARCH=${REF_MARCH:-x86-64-v4}
 
 if [ ! -d "$REF" ]; then
     echo "build_ref.sh: $REF not present; skipping the reference build" >&2
