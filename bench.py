@@ -91,8 +91,27 @@ def make_inputs(rng, roles, sizes):
     return out
 
 
-def gpu_leg(kg, steps: int, warmup: int, rng):
+def calibrate(ctx, kg, reps: int = 5):
+    """Plain copy of 2 x 256 MiB (exactly 512 MiB of HBM traffic per launch):
+    achievable bandwidth, and the yardstick for scaling FETCH_SIZE/WRITE_SIZE."""
+    n = 2 * N
+    src = ctx.empty(n)
+    dst = ctx.empty(n)
+    ctx.call("stream_copy", dst, src, n)     # first touch
+    ctx.sync()
+    ctx.drain()
+    for _ in range(reps):
+        ctx.call("stream_copy", dst, src, n)
+    t = float(np.mean(ctx.drain()))
+    src.free()
+    dst.free()
+    return {"bytes_per_launch": 16 * n, "mean_launch_us": round(t * 1e6, 2),
+            "gbs": round(16 * n / t / 1e9, 1)}
+
+
+def gpu_leg(kg, steps: int, warmup: int, rng, do_calibrate=False):
     ctx = kg.Context(0)
+    cal = calibrate(ctx, kg) if do_calibrate else None
     bufs_by_stencil = {}
     host_pool = {}
     for name, (dims, bpp, coefs, roles, pairs) in STENCILS.items():
@@ -138,7 +157,7 @@ def gpu_leg(kg, steps: int, warmup: int, rng):
             "regs": ctx.regcount(KNAME.get(name, name)),
         }
     ctx.close()
-    return wall, per
+    return wall, per, cal
 
 
 def cpu_leg(reps: int, rng):
@@ -198,7 +217,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also time a plain 512 MiB device copy (reported as copy_calibration)")
+    ap.add_argument("--only", default="",
+                    help="comma-separated subset of stencils (profiling runs)")
     args = ap.parse_args()
+    if args.only:
+        keep = args.only.split(",")
+        for k in list(STENCILS):
+            if k not in keep:
+                del STENCILS[k]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; the stencil path runs on a single "
               "device (multi-GPU z-slab sharding is described in DESIGN.md, not built)",
@@ -206,7 +234,7 @@ def main() -> int:
         return 2
     rng = np.random.default_rng(2024)
     kg = load_pkg()
-    wall, per = gpu_leg(kg, args.steps, args.warmup, rng)
+    wall, per, cal = gpu_leg(kg, args.steps, args.warmup, rng, args.calibrate)
     value = len(STENCILS) * N * args.steps / wall
     dom = max(per, key=lambda k: per[k]["us"])
     d = per[dom]
@@ -236,6 +264,8 @@ def main() -> int:
                      "bytes_per_launch": d["bytes_per_pt"] * N, "mean_launch_us": d["us"]},
         "per_stencil": per,
     }
+    if cal:
+        line["copy_calibration"] = cal
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg(args.cpu_reps, rng)
     print(json.dumps(line), flush=True)

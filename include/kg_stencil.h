@@ -72,6 +72,12 @@ int kg_kernel_regcount(kg_ctx* ctx, const char* kernel, int* vgprs);
 const char* kg_strerror(int status);
 const char* kg_last_error(void);
 
+/* Calibration kernel, no counterpart in the reference: dst[0,n) = src[0,n)
+ * with 16-byte vector accesses (buffers 16-byte aligned).  Its HBM traffic is
+ * exactly 16*n bytes, the yardstick for achievable bandwidth and for scaling
+ * rocprofv3's FETCH_SIZE / WRITE_SIZE counters. */
+int kg_stream_copy(kg_ctx* ctx, double* dst, const double* src, size_t n);
+
 /* ---- FP64 drop-in stencils (one entry per reference kernel) ------------- */
 
 /* laplacian/laplacian.c:44-102.  w1 = alpha*w0 + beta*(6 face neighbours) on

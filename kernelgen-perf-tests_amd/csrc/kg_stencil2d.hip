@@ -68,9 +68,9 @@ void jacobi_k(int nx, int ny, int ntx, double c0, double c1, double c2,
     for (int j = t.j0; j < t.j1; j++) {
 #pragma unroll
         for (int d = 0; d < 3; d++) r[2][d] = w0[o + sy + d - 1];
-        w1[o] = c0 * r[1][1] +
-                c1 * (r[1][0] + r[0][1] + r[1][2] + r[2][1]) +
-                c2 * (r[0][0] + r[2][0] + r[0][2] + r[2][2]);
+        kg_st(&w1[o], c0 * r[1][1] +
+                      c1 * (r[1][0] + r[0][1] + r[1][2] + r[2][1]) +
+                      c2 * (r[0][0] + r[2][0] + r[0][2] + r[2][2]));
 #pragma unroll
         for (int d = 0; d < 3; d++) { r[0][d] = r[1][d]; r[1][d] = r[2][d]; }
         o += sy;
@@ -99,14 +99,14 @@ void gaussblur_k(int nx, int ny, int ntx, double f, double s0, double s1, double
 #pragma unroll
         for (int d = 0; d < 5; d++) r[4][d] = w0[o + 2 * sy + d - 2];
 #define W(di, dj) r[(dj) + 2][(di) + 2]
-        w1[o] = f * (
+        kg_st(&w1[o], f * (
             s0 * W(0, 0) +
             s1 * (W(-1, 0) + W(1, 0) + W(0, -1) + W(0, 1)) +
             s2 * (W(-1, -1) + W(1, -1) + W(-1, 1) + W(1, 1)) +
             s4 * (W(-2, 0) + W(2, 0) + W(0, -2) + W(0, 2)) +
             s5 * (W(-2, -1) + W(-1, -2) + W(1, -2) + W(2, -1) +
                   W(-2, 1) + W(-1, 2) + W(1, 2) + W(2, 1)) +
-            s8 * (W(-2, -2) + W(2, -2) + W(-2, 2) + W(2, 2)));
+            s8 * (W(-2, -2) + W(2, -2) + W(-2, 2) + W(2, 2))));
 #undef W
 #pragma unroll
         for (int m = 0; m < 4; m++)
@@ -139,7 +139,7 @@ void gameoflife_k(int nx, int ny, int ntx, const double* __restrict__ u0, double
             r[0][0] + r[0][1] + r[0][2] +
             r[1][0] + r[1][2] +
             r[2][0] + r[2][1] + r[2][2];
-        u1[o] = 1. / (1. + (r[1][1] + L - 3.) * (L - 3.) * C);
+        kg_st(&u1[o], 1. / (1. + (r[1][1] + L - 3.) * (L - 3.) * C));
 #pragma unroll
         for (int d = 0; d < 3; d++) { r[0][d] = r[1][d]; r[1][d] = r[2][d]; }
         o += sy;
@@ -188,11 +188,11 @@ void whispering_k(int nx, int ny, int ntx, double mu, double epsilon,
         const double ex_src = (i - 1 >= 1) ? ca_x * e1_x - cb_x * dx : e1_x;
         const double ey = ca_m * ey_src + cb_m * dy;
         const double ex = ca_x * ex_src - cb_x * dx;
-        eo0[o] = e0;
-        eo1[o] = e1;
+        kg_st(&eo0[o], e0);
+        kg_st(&eo1[o], e1);
         const double h = da[o] * h_c + db[o] * (e0 - ey + ex - e1);
-        h1[o] = h;
-        u_em1[o] = u_em0[o] + 0.5 * (h * h / mu + epsilon * (e0 * e0 + e1 * e1));
+        kg_st(&h1[o], h);
+        kg_st(&u_em1[o], u_em0[o] + 0.5 * (h * h / mu + epsilon * (e0 * e0 + e1 * e1)));
         h_m = h_c;
         h_c = h_p;
         ca_m = ca_c;

@@ -1,14 +1,18 @@
 // kg_stencil3d.hip -- FP64 3D stencils of the suite: laplacian, divergence,
 // gradient, wave13pt, uxx1, lapgsrb.
 //
-// Each kernel marches a TX x TY column tile through KC z-planes (kg_zmarch.h),
-// keeping the z-column of its main input in registers, so every input array
-// streams through HBM about once per step; xy neighbours come from the
-// L1/L2 lines the neighbouring lanes and rows of the same tile fetch.
+// laplacian, gradient, wave13pt and lapgsrb run on the plane-staged engine
+// (kg_plane.h): xy-neighbours from LDS, z-neighbours from a register column,
+// every global load prefetched a few planes ahead.  divergence and uxx1 read
+// each of their inputs at only a few offsets along one axis and use the plain
+// z-march (kg_zmarch.h): a TX x TY column tile marched through KC planes.
 // Arithmetic is written in the reference's source order and the library is
 // built with -ffp-contract=off: results are bit-identical to an IEEE build of
 // the reference kernels.
+#include "kg_plane.h"
 #include "kg_zmarch.h"
+
+#include <type_traits>
 
 namespace {
 
@@ -18,35 +22,6 @@ constexpr int KC = 16;   // z-planes per workgroup
 
 #define KG_REQUIRE(cond, msg) \
     do { if (!(cond)) return kg_set_error(KG_ERR_INVALID, "%s", msg); } while (0)
-
-// ---------------------------------------------------------------- laplacian
-// laplacian/laplacian.c:93-97
-__global__ __launch_bounds__(TX * TY)
-void laplacian_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double beta,
-                 const double* __restrict__ w0, double* __restrict__ w1)
-{
-    kg_ztile t;
-    if (!kg_ztile_get<TX, TY, KC, 1, 1>(nx, ny, ns, ntx, nty, t)) return;
-    const long long sz = (long long)nx * ny;
-    const int sy = nx;
-    const long long o = kg_off3(t.i, t.j, t.k0, nx, sz);
-    const double* p = w0 + o;
-    double* q = w1 + o;
-    double zm = p[-sz];
-    double c = p[0];
-#pragma unroll 4
-    for (int k = t.k0; k < t.k1; k++) {
-        const double zp = p[sz];
-        q[0] = alpha * c + beta * (
-            p[1] + p[-1] +
-            p[sy] + p[-sy] +
-            zp + zm);
-        zm = c;
-        c = zp;
-        p += sz;
-        q += sz;
-    }
-}
 
 // --------------------------------------------------------------- divergence
 // divergence/divergence.c:93-96
@@ -65,72 +40,11 @@ void divergence_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double
 #pragma unroll 4
     for (int k = t.k0; k < t.k1; k++) {
         const double zp = uz[o + sz];
-        u[o] = alpha * (ux[o + 1] - ux[o - 1]) +
-               beta * (uy[o + sy] - uy[o - sy]) +
-               gamma * (zp - zm);
+        kg_st(&u[o], alpha * (ux[o + 1] - ux[o - 1]) +
+                     beta * (uy[o + sy] - uy[o - sy]) +
+                     gamma * (zp - zm));
         zm = zc;
         zc = zp;
-        o += sz;
-    }
-}
-
-// ----------------------------------------------------------------- gradient
-// gradient/gradient.c:95-97
-__global__ __launch_bounds__(TX * TY)
-void gradient_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double beta, double gamma,
-                const double* __restrict__ u, double* __restrict__ ux,
-                double* __restrict__ uy, double* __restrict__ uz)
-{
-    kg_ztile t;
-    if (!kg_ztile_get<TX, TY, KC, 1, 1>(nx, ny, ns, ntx, nty, t)) return;
-    const long long sz = (long long)nx * ny;
-    const int sy = nx;
-    long long o = kg_off3(t.i, t.j, t.k0, nx, sz);
-    double zm = u[o - sz];
-    double zc = u[o];
-#pragma unroll 4
-    for (int k = t.k0; k < t.k1; k++) {
-        const double zp = u[o + sz];
-        ux[o] = alpha * (u[o + 1] - u[o - 1]);
-        uy[o] = beta * (u[o + sy] - u[o - sy]);
-        uz[o] = gamma * (zp - zm);
-        zm = zc;
-        zc = zp;
-        o += sz;
-    }
-}
-
-// ----------------------------------------------------------------- wave13pt
-// wave13pt/wave13pt.c:640-651 (default path), interior [2,n-2)^3
-__global__ __launch_bounds__(TX * TY)
-void wave13pt_k(int nx, int ny, int ns, int ntx, int nty, double m0, double m1, double m2,
-                const double* __restrict__ w0, const double* __restrict__ w1,
-                double* __restrict__ w2)
-{
-    kg_ztile t;
-    if (!kg_ztile_get<TX, TY, KC, 2, 2>(nx, ny, ns, ntx, nty, t)) return;
-    const long long sz = (long long)nx * ny;
-    const int sy = nx;
-    long long o = kg_off3(t.i, t.j, t.k0, nx, sz);
-    const double* p = w1 + o;
-    double zm2 = p[-2 * sz], zm1 = p[-sz], zc = p[0], zp1 = p[sz];
-#pragma unroll 4
-    for (int k = t.k0; k < t.k1; k++) {
-        const double zp2 = p[2 * sz];
-        w2[o] = m0 * zc - w0[o] +
-            m1 * (
-                p[1] + p[-1] +
-                p[sy] + p[-sy] +
-                zp1 + zm1) +
-            m2 * (
-                p[2] + p[-2] +
-                p[2 * sy] + p[-2 * sy] +
-                zp2 + zm2);
-        zm2 = zm1;
-        zm1 = zc;
-        zc = zp1;
-        zp1 = zp2;
-        p += sz;
         o += sz;
     }
 }
@@ -157,13 +71,13 @@ void uxx1_k(int nx, int ny, int ns, int ntx, int nty, double c1, double c2, doub
         const double d_c = d1[o], d_y = d1[o - sy];
         const double xz_p1 = xz[o + sz];
         const double d = 0.25 * (d_c + d_y + d_c_m + d_y_m);
-        u1[o] = u0[o] + (dth / d) * (
+        kg_st(&u1[o], u0[o] + (dth / d) * (
             c1 * (xx[o] - xx[o - 1]) +
             c2 * (xx[o + 1] - xx[o - 2]) +
             c1 * (xy[o] - xy[o - sy]) +
             c2 * (xy[o + sy] - xy[o - 2 * sy]) +
             c1 * (xz_c - xz_m1) +
-            c2 * (xz_p1 - xz_m2));
+            c2 * (xz_p1 - xz_m2)));
         d_c_m = d_c;
         d_y_m = d_y;
         xz_m2 = xz_m1;
@@ -173,42 +87,137 @@ void uxx1_k(int nx, int ny, int ns, int ntx, int nty, double c1, double c2, doub
     }
 }
 
-// ------------------------------------------------------------------ lapgsrb
-// lapgsrb/lapgsrb.c:93-117, interior [2,n-2)^3
-__global__ __launch_bounds__(TX * TY)
-void lapgsrb_k(int nx, int ny, int ns, int ntx, int nty, double c0, double c1, double c2, double c3,
-               const double* __restrict__ w0, double* __restrict__ w1)
-{
-    kg_ztile t;
-    if (!kg_ztile_get<TX, TY, KC, 2, 2>(nx, ny, ns, ntx, nty, t)) return;
-    const long long sz = (long long)nx * ny;
-    const int sy = nx;
-    long long o = kg_off3(t.i, t.j, t.k0, nx, sz);
-    const double* p = w0 + o;
-    double zm2 = p[-2 * sz], zm1 = p[-sz], zc = p[0], zp1 = p[sz];
-#pragma unroll 2
-    for (int k = t.k0; k < t.k1; k++) {
-        const double zp2 = p[2 * sz];
-        w1[o] = c0 * zc +
-            c1 * (
-                p[1] + p[-1] +
-                p[sy] + p[-sy] +
-                zp1 + zm1) +
-            c2 * (
-                p[1 + sy] + p[1 + sz] + p[1 - sz] + p[1 - sy] +
-                p[sy + sz] + p[sy - sz] + p[-sy + sz] + p[-sy - sz] +
-                p[-1 + sy] + p[-1 + sz] + p[-1 - sz] + p[-1 - sy]) +
-            c3 * (
-                p[2] + p[-2] +
-                p[2 * sy] + p[-2 * sy] +
-                zp2 + zm2);
-        zm2 = zm1;
-        zm1 = zc;
-        zc = zp1;
-        zp1 = zp2;
-        p += sz;
-        o += sz;
+// ------------------------------------------------- plane-engine stencil ops
+// Each op computes the VEC points of one lane at plane k (see kgp::View).
+
+// laplacian/laplacian.c:93-97
+template <class C>
+struct LaplacianOp {
+    const double* src;
+    const double* side[1];
+    double* dst;
+    double alpha, beta;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = alpha * p.col(v, 0) + beta * (
+                p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
+                p.col(v, 1) + p.col(v, -1));
+        kgp::store<C::VEC>(dst, ob, x, r, xm);
     }
+};
+
+// gradient/gradient.c:95-97
+template <class C>
+struct GradientOp {
+    const double* src;
+    const double* side[1];
+    double *ux, *uy, *uz;
+    double alpha, beta, gamma;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double rx[C::VEC], ry[C::VEC], rz[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) {
+            rx[v] = alpha * (p.at(v, 1, 0, 0) - p.at(v, -1, 0, 0));
+            ry[v] = beta * (p.at(v, 0, 1, 0) - p.at(v, 0, -1, 0));
+            rz[v] = gamma * (p.col(v, 1) - p.col(v, -1));
+        }
+        kgp::store<C::VEC>(ux, ob, x, rx, xm);
+        kgp::store<C::VEC>(uy, ob, x, ry, xm);
+        kgp::store<C::VEC>(uz, ob, x, rz, xm);
+    }
+};
+
+// wave13pt/wave13pt.c:640-651; main field w1, side stream w0
+template <class C>
+struct Wave13ptOp {
+    const double* src;
+    const double* side[1];
+    double* dst;
+    double m0, m1, m2;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = m0 * p.col(v, 0) - p.side(0, v) +
+                m1 * (
+                    p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                    p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
+                    p.col(v, 1) + p.col(v, -1)) +
+                m2 * (
+                    p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
+                    p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
+                    p.col(v, 2) + p.col(v, -2));
+        kgp::store<C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
+// lapgsrb/lapgsrb.c:93-117
+template <class C>
+struct LapgsrbOp {
+    const double* src;
+    const double* side[1];
+    double* dst;
+    double c0, c1, c2, c3;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = c0 * p.col(v, 0) +
+                c1 * (
+                    p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                    p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
+                    p.col(v, 1) + p.col(v, -1)) +
+                c2 * (
+                    p.at(v, 1, 1, 0) + p.at(v, 1, 0, 1) + p.at(v, 1, 0, -1) + p.at(v, 1, -1, 0) +
+                    p.at(v, 0, 1, 1) + p.at(v, 0, 1, -1) + p.at(v, 0, -1, 1) + p.at(v, 0, -1, -1) +
+                    p.at(v, -1, 1, 0) + p.at(v, -1, 0, 1) + p.at(v, -1, 0, -1) + p.at(v, -1, -1, 0)) +
+                c3 * (
+                    p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
+                    p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
+                    p.col(v, 2) + p.col(v, -2));
+        kgp::store<C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
+//                 VEC W  H  LL LR ZL ZR D  NAUX KC
+template <int V> using LapCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 3, 0, 32>;
+template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
+template <int V> using GsrbCfg = kgp::Cfg<V, 12, 2, 1, 1, 2, 2, 2, 0, 32>;
+
+// Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx.
+template <template <int> class CfgT, template <class> class OpT, int LO, int HI, class Fill>
+static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, Fill fill)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+    auto go = [&](auto vtag) -> int {
+        constexpr int V = decltype(vtag)::value;
+        using C = CfgT<V>;
+        using Op = OpT<C>;
+        Op op{};
+        fill(op);
+        const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI);
+        auto kern = kgp::plane_kernel<C, LO, HI, Op>;
+        kg_register_kernel(name, (const void*)kern);
+        if (int rc = kg_launch_begin(ctx, name)) return rc;
+        if (!g.empty)
+            hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), 0, ctx->stream,
+                               nx, ny, ns, g.ntx, g.nty, op);
+        return kg_launch_end(ctx);
+    };
+    if (nx % 2 == 0) return go(std::integral_constant<int, 2>{});
+    return go(std::integral_constant<int, 1>{});
 }
 
 }  // namespace
@@ -233,7 +242,12 @@ extern "C" int kg_laplacian(kg_ctx* ctx, int nx, int ny, int ns, double alpha, d
 {
     KG_REQUIRE(w0 && w1, "kg_laplacian: null buffer");
     KG_REQUIRE(w0 != w1, "kg_laplacian: w0 and w1 must be distinct (out-of-place update)");
-    KG_LAUNCH3("laplacian", laplacian_k, 1, 1, alpha, beta, w0, w1);
+    return launch_plane<LapCfg, LaplacianOp, 1, 1>(ctx, "laplacian", nx, ny, ns, [&](auto& op) {
+        op.src = w0;
+        op.dst = w1;
+        op.alpha = alpha;
+        op.beta = beta;
+    });
 }
 
 extern "C" int kg_divergence(kg_ctx* ctx, int nx, int ny, int ns, double alpha, double beta,
@@ -251,7 +265,15 @@ extern "C" int kg_gradient(kg_ctx* ctx, int nx, int ny, int ns, double alpha, do
     KG_REQUIRE(u && ux && uy && uz, "kg_gradient: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz && ux != uy && ux != uz && uy != uz,
                "kg_gradient: buffers must be distinct");
-    KG_LAUNCH3("gradient", gradient_k, 1, 1, alpha, beta, gamma, u, ux, uy, uz);
+    return launch_plane<LapCfg, GradientOp, 1, 1>(ctx, "gradient", nx, ny, ns, [&](auto& op) {
+        op.src = u;
+        op.ux = ux;
+        op.uy = uy;
+        op.uz = uz;
+        op.alpha = alpha;
+        op.beta = beta;
+        op.gamma = gamma;
+    });
 }
 
 extern "C" int kg_wave13pt(kg_ctx* ctx, int nx, int ny, int ns, double m0, double m1, double m2,
@@ -259,7 +281,14 @@ extern "C" int kg_wave13pt(kg_ctx* ctx, int nx, int ny, int ns, double m0, doubl
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt: w2 aliases an input");
-    KG_LAUNCH3("wave13pt", wave13pt_k, 2, 2, m0, m1, m2, w0, w1, w2);
+    return launch_plane<WaveCfg, Wave13ptOp, 2, 2>(ctx, "wave13pt", nx, ny, ns, [&](auto& op) {
+        op.src = w1;
+        op.side[0] = w0;
+        op.dst = w2;
+        op.m0 = m0;
+        op.m1 = m1;
+        op.m2 = m2;
+    });
 }
 
 extern "C" int kg_uxx1(kg_ctx* ctx, int nx, int ny, int ns, double c1, double c2,
@@ -278,5 +307,12 @@ extern "C" int kg_lapgsrb(kg_ctx* ctx, int nx, int ny, int ns, double c0, double
 {
     KG_REQUIRE(w0 && w1, "kg_lapgsrb: null buffer");
     KG_REQUIRE(w0 != w1, "kg_lapgsrb: w0 and w1 must be distinct");
-    KG_LAUNCH3("lapgsrb", lapgsrb_k, 2, 2, c0, c1, c2, c3, w0, w1);
+    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2>(ctx, "lapgsrb", nx, ny, ns, [&](auto& op) {
+        op.src = w0;
+        op.dst = w1;
+        op.c0 = c0;
+        op.c1 = c1;
+        op.c2 = c2;
+        op.c3 = c3;
+    });
 }

@@ -35,6 +35,7 @@ SIGNATURES = {
     "kg_whispering": [_i, _i, _d, _d] + [_dp] * 10,
     "kg_gameoflife": [_i, _i, _dp, _dp],
     "kg_matmul": [_i, _i, _i, _dp, _dp, _dp],
+    "kg_stream_copy": [_dp, _dp, C.c_size_t],
 }
 
 # Every symbol include/kg_stencil.h declares.
