@@ -1,0 +1,66 @@
+/*
+ * gradient -- host program of the test (reference: gradient/gradient.c:114-402).
+ * The kernel is idempotent (same input every step) but runs nt times.
+ */
+#include "../kernelgen-perf-tests_amd/host/kg_host.h"
+
+int main(int argc, char* argv[])
+{
+    if (argc != 5) {
+        printf("Usage: %s <nx> <ny> <ns> <nt>\n", argv[0]);
+        exit(1);
+    }
+    kg_run r;
+    kg_run_begin(&r);
+    KG_PARSE_ARG(nx, argv[1]);
+    KG_PARSE_ARG(ny, argv[2]);
+    KG_PARSE_ARG(ns, argv[3]);
+    KG_PARSE_ARG(nt, argv[4]);
+
+    double alpha = kg_real_rand();
+    double beta = kg_real_rand();
+    double gamma = kg_real_rand();
+    printf("alpha = %f, beta = %f, gamma = %f\n", alpha, beta, gamma);
+
+    size_t n = (size_t)nx * ny * ns;
+    double* u = kg_host_array(n);
+    double* ux = kg_host_array(n);
+    double* uy = kg_host_array(n);
+    double* uz = kg_host_array(n);
+    double mean = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        u[i] = kg_real_rand();
+        ux[i] = kg_real_rand();
+        uy[i] = kg_real_rand();
+        uz[i] = kg_real_rand();
+        mean += u[i] + ux[i] + uy[i] + uz[i];
+    }
+    if (!r.no_timing) printf("initial mean = %f\n", mean / n / 4);
+
+    kg_run_init(&r);
+    double* host[4] = {u, ux, uy, uz};
+    double* dev[4];
+    size_t len[4] = {n, n, n, n};
+    kg_alloc_phase(&r, 4, dev, host, len);
+    /* the reference prints three arrays' worth for four copies (gradient.c:277) */
+    kg_load_phase(&r, 4, dev, host, len, 3.0 * n * sizeof(double));
+
+    kg_phase_start(&r);
+    for (int it = 0; it < nt; it++) {
+        KG_CALL(kg_gradient(r.ctx, nx, ny, ns, alpha, beta, gamma, dev[0], dev[1], dev[2], dev[3]));
+        kg_after_launch(&r);
+    }
+    double t = kg_phase_stop(&r);
+    if (!r.no_timing) printf("compute time = %f sec\n", t);
+
+    /* three copies back, one array's worth printed (gradient.c:344-350) */
+    kg_save_phase(&r, 3, &host[1], &dev[1], len, (double)n * sizeof(double));
+    kg_free_phase(&r, 4, dev);
+
+    double mean_out = 0.0;
+    for (size_t i = 0; i < n; i++) mean_out += ux[i] + uy[i] + uz[i];
+    printf("final mean = %f\n", mean_out / n / 3);
+    for (int i = 0; i < 4; i++) free(host[i]);
+    kg_run_end(&r);
+    return 0;
+}

@@ -1,0 +1,66 @@
+/*
+ * lapgsrb -- host program of the test (reference: lapgsrb/lapgsrb.c:134-408).
+ * w0/w1 ping-pong.
+ */
+#include "../kernelgen-perf-tests_amd/host/kg_host.h"
+
+int main(int argc, char* argv[])
+{
+    if (argc != 5) {
+        printf("Usage: %s <nx> <ny> <ns> <nt>\n", argv[0]);
+        exit(1);
+    }
+    kg_run r;
+    kg_run_begin(&r);
+    KG_PARSE_ARG(nx, argv[1]);
+    KG_PARSE_ARG(ny, argv[2]);
+    KG_PARSE_ARG(ns, argv[3]);
+    KG_PARSE_ARG(nt, argv[4]);
+
+    double c0 = kg_real_rand();
+    double c1 = kg_real_rand() / 6.;
+    double c2 = kg_real_rand() / 12.;
+    double c3 = kg_real_rand() / 6.;
+    printf("c0 = %f, c1 = %f, c2 = %f, c3 = %f\n", c0, c1, c2, c3);
+
+    size_t n = (size_t)nx * ny * ns;
+    double* w0 = kg_host_array(n);
+    double* w1 = kg_host_array(n);
+    double mean = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        w0[i] = kg_real_rand();
+        w1[i] = kg_real_rand();
+        mean += w0[i] + w1[i];
+    }
+    if (!r.no_timing) printf("initial mean = %f\n", mean / n / 2);
+
+    kg_run_init(&r);
+    double* host[2] = {w0, w1};
+    double* dev[2];
+    size_t len[2] = {n, n};
+    kg_alloc_phase(&r, 2, dev, host, len);
+    kg_load_phase(&r, 2, dev, host, len, 2.0 * n * sizeof(double));
+
+    int idxs[2] = {0, 1};
+    kg_phase_start(&r);
+    double *w0p = dev[0], *w1p = dev[1];
+    for (int it = 0; it < nt; it++) {
+        KG_CALL(kg_lapgsrb(r.ctx, nx, ny, ns, c0, c1, c2, c3, w0p, w1p));
+        kg_after_launch(&r);
+        double* w = w0p; w0p = w1p; w1p = w;
+        int idx = idxs[0]; idxs[0] = idxs[1]; idxs[1] = idx;
+    }
+    double t = kg_phase_stop(&r);
+    if (!r.no_timing) printf("compute time = %f sec\n", t);
+
+    double* out_h = host[idxs[1]];
+    double* out_d = dev[idxs[1]];
+    kg_save_phase(&r, 1, &out_h, &out_d, &n, (double)n * sizeof(double));
+    kg_free_phase(&r, 2, dev);
+
+    printf("final mean = %f\n", kg_sum(out_h, n) / n);
+    free(w0);
+    free(w1);
+    kg_run_end(&r);
+    return 0;
+}

@@ -1,0 +1,58 @@
+##===----------------------------------------------------------------------===##
+## Per-target fan-out over the tests (same role as the reference's makefile):
+##   make gcc | make hip         build every test for one target
+##   make <test>.hip             build one test
+##   make test.hip               ./benchmark for the hip target
+##   make benchmark              ./benchmark NX NY NS NT NRUNS hip gcc
+##===----------------------------------------------------------------------===##
+
+-include makefile.in
+
+TARGETS = \
+	divergence \
+	gameoflife \
+	gaussblur \
+	gradient \
+	jacobi \
+	lapgsrb \
+	laplacian \
+	matmul \
+	tricubic \
+	tricubic2 \
+	uxx1 \
+	wave13pt \
+	whispering
+
+.PHONY: all gcc hip lib clean benchmark test.gcc test.hip
+
+all: gcc hip
+
+lib:
+	$(SILENT)$(MAKE) -s -C kernelgen-perf-tests_amd ARCH=$(HIPARCH)
+
+gcc: $(addsuffix .gcc, $(TARGETS))
+
+hip: lib $(addsuffix .hip, $(TARGETS))
+
+%.gcc:
+	$(SILENT)$(MAKE) -s -C $(subst .gcc,,$@)/gcc
+
+%.hip: lib
+	$(SILENT)$(MAKE) -s -C $(subst .hip,,$@)/hip
+
+%.gcc.clean:
+	$(SILENT)$(MAKE) -s -C $(subst .gcc.clean,,$@)/gcc clean
+
+%.hip.clean:
+	$(SILENT)$(MAKE) -s -C $(subst .hip.clean,,$@)/hip clean
+
+clean: $(addsuffix .gcc.clean, $(TARGETS)) $(addsuffix .hip.clean, $(TARGETS))
+
+test.gcc: gcc
+	$(SILENT)./benchmark $(NX) $(NY) $(NS) $(NT) $(NRUNS) gcc
+
+test.hip: hip
+	$(SILENT)./benchmark $(NX) $(NY) $(NS) $(NT) $(NRUNS) hip
+
+benchmark: hip gcc
+	$(SILENT)./benchmark $(NX) $(NY) $(NS) $(NT) $(NRUNS) hip gcc

@@ -113,6 +113,87 @@ def random_case(spec: Spec, dims, seed: int):
     return coefs, arrays
 
 
+# ------------------------------------------------- reference main() replay
+# Which buffers each reference program averages, and how (e.g.
+# laplacian/laplacian.c:159-165 and :374-377).  "final" names the role after
+# the last rotation; the reference's idxs bookkeeping selects the buffer the
+# role points to (for two-level ping-pong that is the input of the last step).
+MEANS = {
+    "laplacian": (("w0", "w1"), 2, ("w1",), 1),
+    "divergence": (("u", "ux", "uy", "uz"), 4, ("u",), 1),
+    "gradient": (("u", "ux", "uy", "uz"), 4, ("ux", "uy", "uz"), 3),
+    "wave13pt": (("w0", "w1", "w2"), 3, ("w1",), 1),
+    "uxx1": (("u0", "u1", "d1", "xx", "xy", "xz"), 6, ("u1",), 1),
+    "tricubic": (("u0", "u1", "a", "b", "c"), 5, ("u1",), 1),
+    "tricubic2": (("u0", "u1", "a", "b", "c"), 5, ("u1",), 1),
+    "lapgsrb": (("w0", "w1"), 2, ("w1",), 1),
+    "jacobi": (("w0", "w1"), 2, ("w1",), 1),
+    "gaussblur": (("w0", "w1"), 2, ("w1",), 1),
+    "gameoflife": (("u0", "u1"), 2, ("u1",), 1),
+}
+
+
+def replay_reference_main(oracle, name: str, dims, nt: int):
+    """Re-run a reference test program's main() with the oracle's kernels:
+    glibc rand() with its default seed, coefficients then arrays drawn in the
+    reference's order, nt steps with its rotation.  Returns (initial mean,
+    final mean, coefficients)."""
+    spec = SPECS[name]
+    lib = oracle.lib
+    lib.oracle_real_rand.restype = C.c_double
+    lib.oracle_srand(C.c_uint(1))
+    coefs = {}
+    for cname, div in zip(spec.coefs, spec.coef_gen or ()):
+        coefs[cname] = lib.oracle_real_rand() / div
+    shapes = array_shapes(spec, dims)
+    arrays = {r: np.empty(n, dtype=np.float64) for r, n in shapes.items()}
+    n = npoints(spec, dims)
+    if name == "matmul":
+        for r in ("A", "B"):
+            a = arrays[r]
+            ptrs = (C.c_void_p * 1)(a.ctypes.data)
+            lib.oracle_fill_interleaved(C.c_size_t(a.size), 1, ptrs)
+        arrays["C"][:] = 0.0
+        i_mean = arrays["A"].mean() + arrays["B"].mean()
+    elif name == "whispering":
+        roles = ("h0", "h1", "u_em0", "u_em1", "ca", "cb", "da", "db")
+        bases = [arrays["e"].ctypes.data, arrays["e"].ctypes.data + 8] + \
+                [arrays[r].ctypes.data for r in roles]
+        strides = [2, 2] + [1] * len(roles)
+        lib.oracle_fill_strided(C.c_size_t(n), len(bases), (C.c_void_p * len(bases))(*bases),
+                                (C.c_long * len(strides))(*strides))
+        i_mean = (arrays["e"].sum() + sum(arrays[r].sum() for r in roles)) / n / 12
+    else:
+        roles = spec.fill
+        ptrs = (C.c_void_p * len(roles))(*[arrays[r].ctypes.data for r in roles])
+        lib.oracle_fill_interleaved(C.c_size_t(n), len(roles), ptrs)
+        mroles, div, _, _ = MEANS[name]
+        i_mean = sum(arrays[r].sum() for r in mroles) / n / div
+    st = dict(arrays)
+    for _ in range(nt):
+        oracle.step(spec, dims, coefs, st)
+        rotate(spec, st)
+    if name == "matmul":
+        f_mean = st["C"].mean()
+    elif name == "whispering":
+        f_mean = (st["e"].sum() + st["h1"].sum() + st["u_em1"].sum()) / n / 4
+    else:
+        _, _, froles, fdiv = MEANS[name]
+        f_mean = sum(st[r].sum() for r in froles) / n / fdiv
+    return float(i_mean), float(f_mean), coefs
+
+
+def driver_format(value: float) -> str:
+    """A mean as the benchmark table shows it: printed %f by the program, then
+    squeezed to 8 characters with %.*g (reference benchmark:27-53)."""
+    v = float("%f" % value)
+    for w in range(6, -1, -1):
+        s = "%.*g" % (w, v)
+        if len(s) <= 8:
+            return s
+    raise ValueError(value)
+
+
 # ------------------------------------------------------------------ rotation
 def rotate(spec: Spec, st: dict) -> None:
     if spec.rotate == "pp2":

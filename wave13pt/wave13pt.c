@@ -1,0 +1,66 @@
+/*
+ * wave13pt -- host program of the test (reference: wave13pt/wave13pt.c:697-1012).
+ * Three time levels rotate w0 <- w1 <- w2 <- w0 after every step (:919-920).
+ */
+#include "../kernelgen-perf-tests_amd/host/kg_host.h"
+
+int main(int argc, char* argv[])
+{
+    if (argc != 5) {
+        printf("Usage: %s <nx> <ny> <ns> <nt>\n", argv[0]);
+        exit(1);
+    }
+    kg_run r;
+    kg_run_begin(&r);
+    KG_PARSE_ARG(nx, argv[1]);
+    KG_PARSE_ARG(ny, argv[2]);
+    KG_PARSE_ARG(ns, argv[3]);
+    KG_PARSE_ARG(nt, argv[4]);
+
+    double m0 = kg_real_rand();
+    double m1 = kg_real_rand() / 6.;
+    double m2 = kg_real_rand() / 6.;
+    printf("m0 = %f, m1 = %f, m2 = %f\n", m0, m1, m2);
+
+    size_t n = (size_t)nx * ny * ns;
+    double* w0 = kg_host_array(n);
+    double* w1 = kg_host_array(n);
+    double* w2 = kg_host_array(n);
+    double mean = 0.0;
+    for (size_t i = 0; i < n; i++) {
+        w0[i] = kg_real_rand();
+        w1[i] = kg_real_rand();
+        w2[i] = kg_real_rand();
+        mean += w0[i] + w1[i] + w2[i];
+    }
+    printf("initial mean = %f\n", mean / n / 3);
+
+    kg_run_init(&r);
+    double* host[3] = {w0, w1, w2};
+    double* dev[3];
+    size_t len[3] = {n, n, n};
+    kg_alloc_phase(&r, 3, dev, host, len);
+    kg_load_phase(&r, 3, dev, host, len, 3.0 * n * sizeof(double));
+
+    int idxs[3] = {0, 1, 2};
+    kg_phase_start(&r);
+    double *w0p = dev[0], *w1p = dev[1], *w2p = dev[2];
+    for (int it = 0; it < nt; it++) {
+        KG_CALL(kg_wave13pt(r.ctx, nx, ny, ns, m0, m1, m2, w0p, w1p, w2p));
+        kg_after_launch(&r);
+        double* w = w0p; w0p = w1p; w1p = w2p; w2p = w;
+        int idx = idxs[0]; idxs[0] = idxs[1]; idxs[1] = idxs[2]; idxs[2] = idx;
+    }
+    double t = kg_phase_stop(&r);
+    if (!r.no_timing) printf("compute time = %f sec\n", t);
+
+    double* out_h = host[idxs[1]];
+    double* out_d = dev[idxs[1]];
+    kg_save_phase(&r, 1, &out_h, &out_d, &n, (double)n * sizeof(double));
+    kg_free_phase(&r, 3, dev);
+
+    printf("final mean = %f\n", kg_sum(out_h, n) / n);
+    for (int i = 0; i < 3; i++) free(host[i]);
+    kg_run_end(&r);
+    return 0;
+}
