@@ -238,10 +238,14 @@ def main() -> int:
     value = len(STENCILS) * N * args.steps / wall
     dom = max(per, key=lambda k: per[k]["us"])
     d = per[dom]
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by the copy kernel).
     traffic = None
-    tfile = ROOT / "profiles" / "traffic.json"
+    tfile = ROOT / "profiles" / "suite_fp64_512x256x256_traffic.json"
     if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get(dom)
+        entry = json.loads(tfile.read_text()).get(dom)
+        if entry:
+            traffic = entry["traffic_bytes"]
     line = {
         "metric": METRIC,
         "value": round(value, 1),

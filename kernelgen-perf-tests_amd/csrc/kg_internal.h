@@ -44,6 +44,16 @@ struct kg_kernel_entry {
 };
 void kg_register_kernel(const char* name, const void* func);
 
+// Device code is loaded lazily, one code object per source file, at the first
+// launch of any of its kernels (about 1 ms).  kg_init calls these so that the
+// load happens in the init phase, not inside the first timed launch.
+int kg_preload_stencil3d();
+int kg_preload_tricubic();
+int kg_preload_stencil2d();
+int kg_preload_matmul();
+int kg_preload_util();
+int kg_preload_kernel(const void* func);
+
 // Validation helpers shared by the entry points.
 int kg_check_ctx(kg_ctx* ctx);
 int kg_check_dims3(int nx, int ny, int ns);

@@ -73,3 +73,5 @@ extern "C" int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, c
     }
     return kg_launch_end(ctx);
 }
+
+int kg_preload_matmul() { return kg_preload_kernel((const void*)matmul_k); }

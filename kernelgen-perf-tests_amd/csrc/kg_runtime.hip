@@ -131,7 +131,22 @@ extern "C" int kg_init(kg_ctx** out, int device)
         return kg_set_error(KG_ERR_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(se));
     }
     c->last_kernel = "";
+    int (*preload[])() = {kg_preload_stencil3d, kg_preload_tricubic, kg_preload_stencil2d,
+                          kg_preload_matmul, kg_preload_util};
+    for (auto f : preload) {
+        if (int rc = f()) {
+            kg_destroy(c);
+            return rc;
+        }
+    }
     *out = c;
+    return KG_OK;
+}
+
+int kg_preload_kernel(const void* func)
+{
+    hipFuncAttributes attr;
+    KG_HIP(hipFuncGetAttributes(&attr, func));
     return KG_OK;
 }
 

@@ -1,231 +1,184 @@
 // kg_stencil2d.hip -- FP64 2D stencils of the suite: jacobi, gaussblur,
 // gameoflife, whispering (the reference runs them at nx x (ny*ns), e.g.
-// 512 x 65536).
-//
-// A workgroup of TX threads (four wave64s side by side along x) marches its
-// x-segment through a chunk of YC interior rows.  Each thread keeps the rows
-// j-r..j+r of its x-neighbourhood in registers and loads one new row per step,
-// so each array streams through HBM about once per step.  Arithmetic is in the
-// reference's source order and the library is built with -ffp-contract=off.
-#include "kg_internal.h"
+// 512 x 65536), on the row-staged engine of kg_row.h: x-neighbours and
+// diagonal neighbours from LDS rows, the lane's own column from registers,
+// every load prefetched a few rows ahead.  Arithmetic is in the reference's
+// source order and the library is built with -ffp-contract=off.
+#include "kg_row.h"
+
+#include <type_traits>
 
 namespace {
 
-constexpr int TX = 256;
-constexpr int YC = 32;
-
-struct ytile {
-    int i, j0, j1;
-};
-
-// Interior [lo, n-hi) on both axes.
-template <int LO, int HI>
-__device__ __forceinline__ bool ytile_get(int nx, int ny, int ntx, ytile& t)
-{
-    const unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
-    const int bx = (int)(b % (unsigned)ntx);
-    const int by = (int)(b / (unsigned)ntx);
-    t.i = bx * TX + (int)threadIdx.x;
-    t.j0 = LO + by * YC;
-    t.j1 = min(t.j0 + YC, ny - HI);
-    return t.i >= LO && t.i < nx - HI && t.j0 < t.j1;
-}
-
-struct ygrid {
-    int ntx, nyc;
-    unsigned blocks;
-    bool empty;
-};
-
-static inline ygrid ygrid_make(int nx, int ny, int lo, int hi)
-{
-    ygrid g;
-    const int nyi = ny - hi - lo;
-    g.empty = (nx - hi - lo <= 0) || nyi <= 0;
-    g.ntx = (nx + TX - 1) / TX;
-    g.nyc = g.empty ? 0 : (nyi + YC - 1) / YC;
-    g.blocks = (unsigned)g.ntx * (unsigned)g.nyc;
-    return g;
-}
-
-// ------------------------------------------------------------------- jacobi
 // jacobi/jacobi.F90:65-67, interior [1,n-1)^2 (Fortran 2..n-1)
-__global__ __launch_bounds__(TX)
-void jacobi_k(int nx, int ny, int ntx, double c0, double c1, double c2,
-              const double* __restrict__ w0, double* __restrict__ w1)
-{
-    ytile t;
-    if (!ytile_get<1, 1>(nx, ny, ntx, t)) return;
-    const long long sy = nx;
-    long long o = (long long)t.j0 * sy + t.i;
-    // r[m][d]: row j+m-1, column i+d-1
-    double r[3][3];
+template <class C>
+struct JacobiOp {
+    const double* src[1];
+    const double* side[1];
+    double* dst;
+    double c0, c1, c2;
+    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r[C::VEC];
 #pragma unroll
-    for (int m = 0; m < 2; m++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) r[m][d] = w0[o + (m - 1) * sy + d - 1];
-#pragma unroll 4
-    for (int j = t.j0; j < t.j1; j++) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) r[2][d] = w0[o + sy + d - 1];
-        kg_st(&w1[o], c0 * r[1][1] +
-                      c1 * (r[1][0] + r[0][1] + r[1][2] + r[2][1]) +
-                      c2 * (r[0][0] + r[2][0] + r[0][2] + r[2][2]));
-#pragma unroll
-        for (int d = 0; d < 3; d++) { r[0][d] = r[1][d]; r[1][d] = r[2][d]; }
-        o += sy;
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = c0 * p.at(0, v, 0, 0) +
+                   c1 * (p.at(0, v, -1, 0) + p.at(0, v, 0, -1) + p.at(0, v, 1, 0) + p.at(0, v, 0, 1)) +
+                   c2 * (p.at(0, v, -1, -1) + p.at(0, v, -1, 1) + p.at(0, v, 1, -1) + p.at(0, v, 1, 1));
+        kgr::store<C::VEC>(dst, o, x, r, xm);
     }
-}
+};
 
-// ---------------------------------------------------------------- gaussblur
-// gaussblur/gaussblur.c:85-92, interior [2,n-2)^2; f from :65 (host-computed)
-__global__ __launch_bounds__(TX)
-void gaussblur_k(int nx, int ny, int ntx, double f, double s0, double s1, double s2,
-                 double s4, double s5, double s8,
-                 const double* __restrict__ w0, double* __restrict__ w1)
-{
-    ytile t;
-    if (!ytile_get<2, 2>(nx, ny, ntx, t)) return;
-    const long long sy = nx;
-    long long o = (long long)t.j0 * sy + t.i;
-    // r[m][d]: row j+m-2, column i+d-2
-    double r[5][5];
+// gaussblur/gaussblur.c:85-92, interior [2,n-2)^2; f (:65) computed on the host
+template <class C>
+struct GaussblurOp {
+    const double* src[1];
+    const double* side[1];
+    double* dst;
+    double f, s0, s1, s2, s4, s5, s8;
+    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r[C::VEC];
 #pragma unroll
-    for (int m = 0; m < 4; m++)
-#pragma unroll
-        for (int d = 0; d < 5; d++) r[m][d] = w0[o + (m - 2) * sy + d - 2];
-#pragma unroll 2
-    for (int j = t.j0; j < t.j1; j++) {
-#pragma unroll
-        for (int d = 0; d < 5; d++) r[4][d] = w0[o + 2 * sy + d - 2];
-#define W(di, dj) r[(dj) + 2][(di) + 2]
-        kg_st(&w1[o], f * (
-            s0 * W(0, 0) +
-            s1 * (W(-1, 0) + W(1, 0) + W(0, -1) + W(0, 1)) +
-            s2 * (W(-1, -1) + W(1, -1) + W(-1, 1) + W(1, 1)) +
-            s4 * (W(-2, 0) + W(2, 0) + W(0, -2) + W(0, 2)) +
-            s5 * (W(-2, -1) + W(-1, -2) + W(1, -2) + W(2, -1) +
-                  W(-2, 1) + W(-1, 2) + W(1, 2) + W(2, 1)) +
-            s8 * (W(-2, -2) + W(2, -2) + W(-2, 2) + W(2, 2))));
+        for (int v = 0; v < C::VEC; v++) {
+#define W(di, dj) p.at(0, v, di, dj)
+            r[v] = f * (
+                s0 * W(0, 0) +
+                s1 * (W(-1, 0) + W(1, 0) + W(0, -1) + W(0, 1)) +
+                s2 * (W(-1, -1) + W(1, -1) + W(-1, 1) + W(1, 1)) +
+                s4 * (W(-2, 0) + W(2, 0) + W(0, -2) + W(0, 2)) +
+                s5 * (W(-2, -1) + W(-1, -2) + W(1, -2) + W(2, -1) +
+                      W(-2, 1) + W(-1, 2) + W(1, 2) + W(2, 1)) +
+                s8 * (W(-2, -2) + W(2, -2) + W(-2, 2) + W(2, 2)));
 #undef W
-#pragma unroll
-        for (int m = 0; m < 4; m++)
-#pragma unroll
-            for (int d = 0; d < 5; d++) r[m][d] = r[m + 1][d];
-        o += sy;
+        }
+        kgr::store<C::VEC>(dst, o, x, r, xm);
     }
-}
+};
 
-// --------------------------------------------------------------- gameoflife
 // gameoflife/gameoflife.c:82-91, interior [1,n-1)^2
-__global__ __launch_bounds__(TX)
-void gameoflife_k(int nx, int ny, int ntx, const double* __restrict__ u0, double* __restrict__ u1)
-{
-    ytile t;
-    if (!ytile_get<1, 1>(nx, ny, ntx, t)) return;
-    const long long sy = nx;
-    long long o = (long long)t.j0 * sy + t.i;
-    double r[3][3];
+template <class C>
+struct GameoflifeOp {
+    const double* src[1];
+    const double* side[1];
+    double* dst;
+    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r[C::VEC];
 #pragma unroll
-    for (int m = 0; m < 2; m++)
-#pragma unroll
-        for (int d = 0; d < 3; d++) r[m][d] = u0[o + (m - 1) * sy + d - 1];
-    const double C = 100000000000000000000.;
-#pragma unroll 4
-    for (int j = t.j0; j < t.j1; j++) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) r[2][d] = u0[o + sy + d - 1];
-        const double L =
-            r[0][0] + r[0][1] + r[0][2] +
-            r[1][0] + r[1][2] +
-            r[2][0] + r[2][1] + r[2][2];
-        kg_st(&u1[o], 1. / (1. + (r[1][1] + L - 3.) * (L - 3.) * C));
-#pragma unroll
-        for (int d = 0; d < 3; d++) { r[0][d] = r[1][d]; r[1][d] = r[2][d]; }
-        o += sy;
+        for (int v = 0; v < C::VEC; v++) {
+#define U(di, dj) p.at(0, v, di, dj)
+            const double Cbig = 100000000000000000000.;
+            const double L =
+                U(-1, -1) + U(0, -1) + U(1, -1) +
+                U(-1, 0) + U(1, 0) +
+                U(-1, 1) + U(0, 1) + U(1, 1);
+            r[v] = 1. / (1. + (U(0, 0) + L - 3.) * (L - 3.) * Cbig);
+#undef U
+        }
+        kgr::store<C::VEC>(dst, o, x, r, xm);
     }
-}
+};
 
-// --------------------------------------------------------------- whispering
 // whispering/whispering.c:94-106 with the serial sweep's in-place e updates
 // made explicit.  Visiting (i,j), the reference reads e(i,j-1,0) after the
 // (i,j-1) update replaced it (when j-1 >= 1) and e(i-1,j,1) after the (i-1,j)
 // update replaced it (when i-1 >= 1).  Those updates are recomputed here from
 // e_in; their difference terms h0(i,j)-h0(i,j-1) and h0(i,j)-h0(i-1,j) are the
 // ones ey and ex use.  e is planar: component c at e[i + nx*j + nx*ny*c].
-__global__ __launch_bounds__(TX)
-void whispering_k(int nx, int ny, int ntx, double mu, double epsilon,
-                  const double* __restrict__ e_in, double* __restrict__ e_out,
-                  const double* __restrict__ h0, double* __restrict__ h1,
-                  const double* __restrict__ u_em0, double* __restrict__ u_em1,
-                  const double* __restrict__ ca, const double* __restrict__ cb,
-                  const double* __restrict__ da, const double* __restrict__ db)
-{
-    ytile t;
-    if (!ytile_get<1, 1>(nx, ny, ntx, t)) return;
-    const long long sy = nx;
-    const long long n = (long long)nx * ny;
-    const double* ei0 = e_in;
-    const double* ei1 = e_in + n;
-    double* eo0 = e_out;
-    double* eo1 = e_out + n;
-    const int i = t.i;
-    long long o = (long long)t.j0 * sy + i;
-    // Row j-1 values of this column carried from the previous step.
-    double h_m = h0[o - sy], ca_m = ca[o - sy], cb_m = cb[o - sy], e0_m = ei0[o - sy];
-    double h_c = h0[o];
-#pragma unroll 2
-    for (int j = t.j0; j < t.j1; j++) {
-        const double ca_c = ca[o], cb_c = cb[o], e0_c = ei0[o], e1_c = ei1[o];
-        const double h_p = h0[o + sy];
-        const double h_xp = h0[o + 1], h_xm = h0[o - 1];
-        const double ca_x = ca[o - 1], cb_x = cb[o - 1], e1_x = ei1[o - 1];
-        const double e0 = ca_c * e0_c + cb_c * (h_p - h_c);
-        const double e1 = ca_c * e1_c - cb_c * (h_xp - h_c);
-        const double dy = h_c - h_m;
-        const double dx = h_c - h_xm;
-        const double ey_src = (j - 1 >= 1) ? ca_m * e0_m + cb_m * dy : e0_m;
-        const double ex_src = (i - 1 >= 1) ? ca_x * e1_x - cb_x * dx : e1_x;
-        const double ey = ca_m * ey_src + cb_m * dy;
-        const double ex = ca_x * ex_src - cb_x * dx;
-        kg_st(&eo0[o], e0);
-        kg_st(&eo1[o], e1);
-        const double h = da[o] * h_c + db[o] * (e0 - ey + ex - e1);
-        kg_st(&h1[o], h);
-        kg_st(&u_em1[o], u_em0[o] + 0.5 * (h * h / mu + epsilon * (e0 * e0 + e1 * e1)));
-        h_m = h_c;
-        h_c = h_p;
-        ca_m = ca_c;
-        cb_m = cb_c;
-        e0_m = e0_c;
-        o += sy;
+// Fields: 0 h0, 1 ca, 2 cb, 3 e_in component 0, 4 e_in component 1;
+// side streams: 0 da, 1 db, 2 u_em0.
+template <class C>
+struct WhisperingOp {
+    const double* src[5];
+    const double* side[3];
+    double *eo0, *eo1, *h1, *u_em1;
+    double mu, epsilon;
+    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        double r0[C::VEC], r1[C::VEC], rh[C::VEC], ru[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) {
+            const int i = x + v;
+            const double h_c = p.col_(0, v, 0), h_p = p.col_(0, v, 1), h_m = p.col_(0, v, -1);
+            const double h_xp = p.at(0, v, 1, 0), h_xm = p.at(0, v, -1, 0);
+            const double ca_c = p.col_(1, v, 0), cb_c = p.col_(2, v, 0);
+            const double e0_c = p.col_(3, v, 0), e1_c = p.col_(4, v, 0);
+            const double ca_m = p.col_(1, v, -1), cb_m = p.col_(2, v, -1), e0_m = p.col_(3, v, -1);
+            const double ca_x = p.at(1, v, -1, 0), cb_x = p.at(2, v, -1, 0), e1_x = p.at(4, v, -1, 0);
+            const double e0 = ca_c * e0_c + cb_c * (h_p - h_c);
+            const double e1 = ca_c * e1_c - cb_c * (h_xp - h_c);
+            const double dy = h_c - h_m;
+            const double dx = h_c - h_xm;
+            const double ey_src = (p.j - 1 >= 1) ? ca_m * e0_m + cb_m * dy : e0_m;
+            const double ex_src = (i - 1 >= 1) ? ca_x * e1_x - cb_x * dx : e1_x;
+            const double ey = ca_m * ey_src + cb_m * dy;
+            const double ex = ca_x * ex_src - cb_x * dx;
+            const double h = p.side(0, v) * h_c + p.side(1, v) * (e0 - ey + ex - e1);
+            r0[v] = e0;
+            r1[v] = e1;
+            rh[v] = h;
+            ru[v] = p.side(2, v) + 0.5 * (h * h / mu + epsilon * (e0 * e0 + e1 * e1));
+        }
+        kgr::store<C::VEC>(eo0, o, x, r0, xm);
+        kgr::store<C::VEC>(eo1, o, x, r1, xm);
+        kgr::store<C::VEC>(h1, o, x, rh, xm);
+        kgr::store<C::VEC>(u_em1, o, x, ru, xm);
     }
+};
+
+//                     VEC W      H  NF LL LR ZL ZR D  NAUX YC
+template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 3, 0, 32>;
+template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 3, 0, 32>;
+template <int V> using WhCfg = kgr::Cfg<V, 8 / V, 1, 5, 0, 0, 1, 1, 2, 3, 32>;
+
+// Launch Op on the row engine; 16-byte lanes (VEC=2) need an even nx.
+template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int VEC_EVEN,
+          class Fill>
+static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, Fill fill)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (int rc = kg_check_dims2(nx, ny)) return rc;
+    auto go = [&](auto vtag) -> int {
+        constexpr int V = decltype(vtag)::value;
+        using C = CfgT<V>;
+        using Op = OpT<C>;
+        Op op{};
+        fill(op);
+        const kgr::Grid g = kgr::make_grid<C>(nx, ny, LO, HI);
+        auto kern = kgr::row_kernel<C, LO, HI, Op>;
+        kg_register_kernel(name, (const void*)kern);
+        if (int rc = kg_launch_begin(ctx, name)) return rc;
+        if (!g.empty)
+            hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), 0, ctx->stream, nx, ny,
+                               g.ntx, op);
+        return kg_launch_end(ctx);
+    };
+    if (VEC_EVEN == 2 && nx % 2 == 0) return go(std::integral_constant<int, VEC_EVEN>{});
+    return go(std::integral_constant<int, 1>{});
 }
-
-}  // namespace
-
-#define KG_LAUNCH2(NAME, KERNEL, LO, HI, ...)                                              \
-    do {                                                                                   \
-        if (int rc_ = kg_check_ctx(ctx)) return rc_;                                       \
-        if (int rc_ = kg_check_dims2(nx, ny)) return rc_;                                  \
-        const ygrid g = ygrid_make(nx, ny, LO, HI);                                        \
-        kg_register_kernel(NAME, (const void*)KERNEL);                                     \
-        if (int rc_ = kg_launch_begin(ctx, NAME)) return rc_;                              \
-        if (!g.empty)                                                                      \
-            hipLaunchKernelGGL(KERNEL, dim3(g.blocks), dim3(TX), 0, ctx->stream,           \
-                               nx, ny, g.ntx, __VA_ARGS__);                                \
-        return kg_launch_end(ctx);                                                         \
-    } while (0)
 
 #define KG_REQUIRE(cond, msg) \
     do { if (!(cond)) return kg_set_error(KG_ERR_INVALID, "%s", msg); } while (0)
+
+}  // namespace
 
 extern "C" int kg_jacobi(kg_ctx* ctx, int nx, int ny, double c0, double c1, double c2,
                          const double* w0, double* w1)
 {
     KG_REQUIRE(w0 && w1, "kg_jacobi: null buffer");
     KG_REQUIRE(w0 != w1, "kg_jacobi: w0 and w1 must be distinct");
-    KG_LAUNCH2("jacobi_", jacobi_k, 1, 1, c0, c1, c2, w0, w1);
+    return launch_row<J9Cfg, JacobiOp, 1, 1, 2>(ctx, "jacobi_", nx, ny, [&](auto& op) {
+        op.src[0] = w0;
+        op.dst = w1;
+        op.c0 = c0;
+        op.c1 = c1;
+        op.c2 = c2;
+    });
 }
 
 extern "C" int kg_gaussblur(kg_ctx* ctx, int nx, int ny, double s0, double s1, double s2,
@@ -234,14 +187,27 @@ extern "C" int kg_gaussblur(kg_ctx* ctx, int nx, int ny, double s0, double s1, d
     KG_REQUIRE(w0 && w1, "kg_gaussblur: null buffer");
     KG_REQUIRE(w0 != w1, "kg_gaussblur: w0 and w1 must be distinct");
     const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
-    KG_LAUNCH2("gaussblur", gaussblur_k, 2, 2, f, s0, s1, s2, s4, s5, s8, w0, w1);
+    return launch_row<G25Cfg, GaussblurOp, 2, 2, 2>(ctx, "gaussblur", nx, ny, [&](auto& op) {
+        op.src[0] = w0;
+        op.dst = w1;
+        op.f = f;
+        op.s0 = s0;
+        op.s1 = s1;
+        op.s2 = s2;
+        op.s4 = s4;
+        op.s5 = s5;
+        op.s8 = s8;
+    });
 }
 
 extern "C" int kg_gameoflife(kg_ctx* ctx, int nx, int ny, const double* u0, double* u1)
 {
     KG_REQUIRE(u0 && u1, "kg_gameoflife: null buffer");
     KG_REQUIRE(u0 != u1, "kg_gameoflife: u0 and u1 must be distinct");
-    KG_LAUNCH2("gameoflife", gameoflife_k, 1, 1, u0, u1);
+    return launch_row<J9Cfg, GameoflifeOp, 1, 1, 2>(ctx, "gameoflife", nx, ny, [&](auto& op) {
+        op.src[0] = u0;
+        op.dst = u1;
+    });
 }
 
 extern "C" int kg_whispering(kg_ctx* ctx, int nx, int ny, double mu, double epsilon,
@@ -254,6 +220,26 @@ extern "C" int kg_whispering(kg_ctx* ctx, int nx, int ny, double mu, double epsi
                "kg_whispering: null buffer");
     KG_REQUIRE(e_in != e_out && h0 != h1 && u_em0 != u_em1,
                "kg_whispering: e, h and u_em need distinct in/out buffers");
-    KG_LAUNCH2("whispering", whispering_k, 1, 1, mu, epsilon, e_in, e_out, h0, h1, u_em0, u_em1,
-               ca, cb, da, db);
+    const long long n = (long long)nx * ny;
+    return launch_row<WhCfg, WhisperingOp, 1, 1, 1>(ctx, "whispering", nx, ny, [&](auto& op) {
+        op.src[0] = h0;
+        op.src[1] = ca;
+        op.src[2] = cb;
+        op.src[3] = e_in;
+        op.src[4] = e_in + n;
+        op.side[0] = da;
+        op.side[1] = db;
+        op.side[2] = u_em0;
+        op.eo0 = e_out;
+        op.eo1 = e_out + n;
+        op.h1 = h1;
+        op.u_em1 = u_em1;
+        op.mu = mu;
+        op.epsilon = epsilon;
+    });
+}
+
+int kg_preload_stencil2d()
+{
+    return kg_preload_kernel((const void*)kgr::row_kernel<J9Cfg<2>, 1, 1, JacobiOp<J9Cfg<2>>>);
 }

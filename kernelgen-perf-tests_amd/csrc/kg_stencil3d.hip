@@ -316,3 +316,5 @@ extern "C" int kg_lapgsrb(kg_ctx* ctx, int nx, int ny, int ns, double c0, double
         op.c3 = c3;
     });
 }
+
+int kg_preload_stencil3d() { return kg_preload_kernel((const void*)divergence_k); }

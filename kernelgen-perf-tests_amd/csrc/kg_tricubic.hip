@@ -131,3 +131,5 @@ extern "C" int kg_tricubic2(kg_ctx* ctx, int nx, int ny, int ns, const double* u
 {
     return launch_tricubic<2, 2, true>("tricubic2", ctx, nx, ny, ns, u0, u1, a, b, c);
 }
+
+int kg_preload_tricubic() { return kg_preload_kernel((const void*)tricubic_k<1, 2, false>); }

@@ -42,3 +42,5 @@ extern "C" int kg_stream_copy(kg_ctx* ctx, double* dst, const double* src, size_
                            (long long)n);
     return kg_launch_end(ctx);
 }
+
+int kg_preload_util() { return kg_preload_kernel((const void*)stream_copy_k); }
