@@ -131,10 +131,12 @@ struct WhisperingOp {
     }
 };
 
+// Row chunks from a sweep at 512x65536 (cold caches, 48-launch means): jacobi
+// YC=64 115 us (vs 118 at 32); whispering YC=8 612 us (vs 650 at 32).
 //                     VEC W      H  NF LL LR ZL ZR D  NAUX YC
-template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 3, 0, 32>;
+template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 3, 0, 64>;
 template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 3, 0, 32>;
-template <int V> using WhCfg = kgr::Cfg<V, 8 / V, 1, 5, 0, 0, 1, 1, 2, 3, 32>;
+template <int V> using WhCfg = kgr::Cfg<V, 8 / V, 1, 5, 0, 0, 1, 1, 2, 3, 8>;
 
 // Launch Op on the row engine; 16-byte lanes (VEC=2) need an even nx.
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int VEC_EVEN,
