@@ -143,12 +143,104 @@ int kg_whispering(kg_ctx* ctx, int nx, int ny, double mu, double epsilon,
  * [1,n-1)^2. */
 int kg_gameoflife(kg_ctx* ctx, int nx, int ny, const double* u0, double* u1);
 
+/* gameoflife/gameoflife.c:82-91 on uint8 cells (X mode): c[i + nx*j], nonzero
+ * = alive; c1 = Conway's rule (alive next iff 3 neighbours, or 2 and alive)
+ * on [1,n-1)^2, written as 0/1; the border of c1 is left as it was.  For 0/1
+ * cells this is the FP64 kernel's rule exactly.  Buffers 16-byte aligned. */
+int kg_life_u8(kg_ctx* ctx, int nx, int ny, const uint8_t* c0, uint8_t* c1);
+
+/* ---- precision variants (beside the FP64 drop-in kernels) -------------------
+ * No reference counterpart: the reference computes in FP64 only.  Same
+ * stencils, interiors and source order as the FP64 entry points above, with
+ * FP32 storage and FP32 arithmetic (_f32) or bfloat16 storage -- uint16_t bit
+ * patterns -- and FP32 arithmetic (_bf16, results rounded to nearest even).
+ * Means, norms and errors of such fields are taken in FP64 with
+ * kg_field_stats. */
+typedef enum { KG_F64 = 0, KG_F32 = 1, KG_BF16 = 2 } kg_dtype;
+
+/* laplacian/laplacian.c:93-97 */
+int kg_laplacian_f32(kg_ctx* ctx, int nx, int ny, int ns, float alpha, float beta,
+                     const float* w0, float* w1);
+int kg_laplacian_bf16(kg_ctx* ctx, int nx, int ny, int ns, float alpha, float beta,
+                      const uint16_t* w0, uint16_t* w1);
+/* wave13pt/wave13pt.c:640-651 */
+int kg_wave13pt_f32(kg_ctx* ctx, int nx, int ny, int ns, float m0, float m1, float m2,
+                    const float* w0, const float* w1, float* w2);
+int kg_wave13pt_bf16(kg_ctx* ctx, int nx, int ny, int ns, float m0, float m1, float m2,
+                     const uint16_t* w0, const uint16_t* w1, uint16_t* w2);
+/* gaussblur/gaussblur.c:65,85-92 applied to every xy-plane of a 3D grid:
+ * xy-interior [2,n-2), all z-planes. */
+int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0, double s1, double s2,
+                      double s4, double s5, double s8, const double* w0, double* w1);
+int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, float s1, float s2,
+                      float s4, float s5, float s8, const float* w0, float* w1);
+
+/* FP64 statistics of a device field x of n values of type dtype, optionally
+ * against a reference field ref of type ref_dtype (NULL: none).  out (host,
+ * 5 doubles) = { sum x, sum x^2, max |x|, sum (x-ref)^2, max |x-ref| }.
+ * Deterministic (fixed reduction order).  Synchronises the context stream. */
+int kg_field_stats(kg_ctx* ctx, int dtype, size_t n, const void* x, int ref_dtype,
+                   const void* ref, double* out);
+
 /* matmul/matmul.F90:14-70, called as matmul_(&nx,&ny,&ns,A,B,C)
  * (matmul/main.c:235).  C(nx,ns) += A(nx,ny)*B(ny,ns), column-major, on the
  * FP64 matrix cores.  Products are accumulated with fused multiply-adds in
  * blocks of k, so results differ from the sequential-k reference by rounding
  * (bounded norm-wise, not bit-exact). */
 int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, const double* B, double* C);
+
+/* ---- batched grids -----------------------------------------------------------
+ * No reference counterpart (the reference runs one grid per process).  A batch
+ * keeps many independent 3D grids resident in HBM and advances all of them by
+ * one step per persistent launch; kg_batch_step(b, n, 1) captures n launches
+ * in a HIP graph and replays it.  Each grid's result is bit-identical to
+ * stepping it alone with the single-grid entry point of its kind and type
+ * (kg_laplacian / _f32, kg_wave13pt / _f32, kg_gaussblur3_f64 / _f32).
+ *
+ * Levels: laplacian and gaussblur3 grids have 2 (ping-pong), wave13pt 3
+ * (w0, w1, w2).  Before the first step, level 0 is the input (wave13pt: w0,
+ * and level 1 is w1); upload every level, since boundary cells are never
+ * written and keep what each level held.  kg_batch_newest_level tells which
+ * level holds the latest values.  Host buffers are dense (x fastest);
+ * device rows are padded to 16 bytes. */
+typedef enum { KG_KIND_LAPLACIAN = 0, KG_KIND_WAVE13PT = 1, KG_KIND_GAUSSBLUR3 = 2 } kg_kind;
+
+typedef struct kg_grid_spec {
+    int kind;        /* kg_kind */
+    int dtype;       /* KG_F64 or KG_F32 */
+    int nx, ny, ns;
+    double coef[6];  /* laplacian: alpha, beta; wave13pt: m0, m1, m2;
+                        gaussblur3: s0, s1, s2, s4, s5, s8 */
+} kg_grid_spec;
+
+typedef struct kg_batch kg_batch;
+
+int kg_batch_create(kg_ctx* ctx, kg_batch** out);
+int kg_batch_destroy(kg_batch* b);
+/* Adds a grid (its levels are allocated, contents undefined until uploaded);
+ * it joins at the batch's current step. */
+int kg_batch_add(kg_batch* b, const kg_grid_spec* spec, int* id);
+/* Releases a grid's levels for reuse by later grids (waits for queued work). */
+int kg_batch_remove(kg_batch* b, int id);
+int kg_batch_upload(kg_batch* b, int id, int level, const void* host);
+int kg_batch_download(kg_batch* b, int id, int level, void* host);
+int kg_batch_newest_level(kg_batch* b, int id, int* level);
+int kg_batch_steps_done(kg_batch* b, int id, long long* steps);
+/* n steps of every grid; use_graph != 0 replays a captured graph of n launches. */
+int kg_batch_step(kg_batch* b, int nsteps, int use_graph);
+/* Grid points and algorithmic HBM bytes per step, work items, workgroups. */
+int kg_batch_info(kg_batch* b, double* points_per_step, double* bytes_per_step,
+                  int* work_items, int* workgroups);
+/* Autotuner: for every size class present (kind, type, nx and ny bucket),
+ * times each tile shape (16/32/64 lanes per row x 8/16/32/64 planes per tile)
+ * over reps launches on the batch's own grids and keeps the fastest.  Grid
+ * contents are preserved. */
+int kg_batch_tune(kg_batch* b, int reps, int* classes_tuned);
+int kg_batch_grid_shape(kg_batch* b, int id, int* lanes_per_row, int* planes_per_tile);
+/* Forces the tile shape of the size class of grid id. */
+int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int planes_per_tile);
+/* HBM reserved by the batch's arena and bytes held by live grids. */
+int kg_batch_memory(kg_batch* b, size_t* reserved, size_t* in_use);
 
 #ifdef __cplusplus
 }

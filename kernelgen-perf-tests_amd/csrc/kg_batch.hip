@@ -1,0 +1,797 @@
+// kg_batch.hip -- many independent grids stepped by one persistent launch.
+//
+// A batch owns a set of 3D grids of mixed kind (laplacian, wave13pt,
+// gaussblur per xy-plane), storage type (FP64, FP32) and size.  Their levels
+// live in HBM for the life of the batch (rows padded to 16 bytes) and are
+// carved from an arena, so adding and removing grids never calls hipFree on
+// the hot path.  One step of every grid is ONE launch of batch_k:
+//
+//   * the work list is every (grid, tile) pair, tiles cut per grid with the
+//     tile shape of its size class (lanes per row 16/32/64 and planes per
+//     tile), largest grids first;
+//   * each resident workgroup pulls the next item from an atomic counter and
+//     runs the plane engine's tile body for that grid's kind and type -- the
+//     same bodies as the single-grid kernels, so a grid's result is bit for
+//     bit what the single-grid entry point gives;
+//   * the level a grid reads and writes follows from a device-resident step
+//     counter, which the last workgroup to finish advances (and the work
+//     counter is reset there), so a captured graph of K launches replays
+//     correctly at any position of the time loop.
+//
+// The host API is C (kg_batch_* in include/kg_stencil.h).  kg_batch_tune
+// times the candidate tile shapes of every size class present on the batch's
+// own grids and keeps the fastest.
+#include "kg_ops3d.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+namespace {
+
+using namespace kgops;
+
+// ------------------------------------------------------------ variants
+constexpr int BW = 16;          // waves per workgroup (1024 threads)
+constexpr int BD = 2;           // prefetch depth
+constexpr int NKIND = 3;        // KG_KIND_LAPLACIAN, _WAVE13PT, _GAUSSBLUR3
+constexpr int NTYPE = 2;        // KG_F64, KG_F32
+constexpr int NLPR = 3;         // lanes per row: 64, 32, 16
+constexpr int NVAR = NKIND * NTYPE * NLPR;
+constexpr int LPRS[NLPR] = {64, 32, 16};
+constexpr int KCS[] = {8, 16, 32, 64};
+constexpr int NKC = sizeof(KCS) / sizeof(KCS[0]);
+
+template <int K> struct Kind;
+template <> struct Kind<KG_KIND_LAPLACIAN> {
+    using B = Bounds<LaplacianOp>;
+    using Sh = Shape<LaplacianOp>;
+    template <class C> using Op = LaplacianOp<C>;
+};
+template <> struct Kind<KG_KIND_WAVE13PT> {
+    using B = Bounds<Wave13ptOp>;
+    using Sh = Shape<Wave13ptOp>;
+    template <class C> using Op = Wave13ptOp<C>;
+};
+template <> struct Kind<KG_KIND_GAUSSBLUR3> {
+    using B = Bounds<Gaussblur3Op>;
+    using Sh = Shape<Gaussblur3Op>;
+    template <class C> using Op = Gaussblur3Op<C>;
+};
+
+template <int DT> struct TypeOf;
+template <> struct TypeOf<KG_F64> { using T = kgp::F64; static constexpr int VEC = 2; };
+template <> struct TypeOf<KG_F32> { using T = kgp::F32; static constexpr int VEC = 4; };
+
+template <int K, int DT, int LPR>
+using VarCfg = kgp::Cfg<TypeOf<DT>::VEC, BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind<K>::Sh::LR,
+                        Kind<K>::Sh::ZL, Kind<K>::Sh::ZR, BD, Kind<K>::Sh::NAUX, 32,
+                        typename TypeOf<DT>::T, LPR>;
+
+constexpr int var_id(int kind, int dt, int li) { return (kind * NTYPE + dt) * NLPR + li; }
+
+// Shape facts of every variant, for the host (tile counts) and the LDS size.
+struct VarInfo {
+    int txw, ty, lo, hi, zlo, zhi, vec, esize;
+    size_t lds;
+};
+
+template <int K, int DT, int LI>
+constexpr VarInfo var_info()
+{
+    using C = VarCfg<K, DT, LPRS[LI]>;
+    using B = typename Kind<K>::B;
+    return VarInfo{C::TXW, C::TY, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC, (int)sizeof(typename C::S),
+                   C::LDS_BYTES};
+}
+
+#define KG_VAR_LIST(X)                                                       \
+    X(0, 0, 0) X(0, 0, 1) X(0, 0, 2) X(0, 1, 0) X(0, 1, 1) X(0, 1, 2)        \
+    X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 1, 0) X(1, 1, 1) X(1, 1, 2)        \
+    X(2, 0, 0) X(2, 0, 1) X(2, 0, 2) X(2, 1, 0) X(2, 1, 1) X(2, 1, 2)
+
+#define KG_VAR_INFO(K, DT, LI) var_info<K, DT, LI>(),
+constexpr VarInfo VARS[NVAR] = {KG_VAR_LIST(KG_VAR_INFO)};
+
+constexpr size_t max_lds()
+{
+    size_t m = 0;
+    for (int v = 0; v < NVAR; v++) m = VARS[v].lds > m ? VARS[v].lds : m;
+    return m;
+}
+constexpr size_t BATCH_LDS = max_lds();
+
+// ------------------------------------------------------- device structures
+struct GridDesc {
+    int var;              // variant id
+    int nx, ny, ns;
+    int kc;               // planes per tile
+    int nlev;             // 2 (laplacian, gaussblur) or 3 (wave13pt)
+    long long sy, sz;     // row and plane pitch (elements)
+    long long step0;      // device step at which this grid's local step 0 runs
+    void* lev[3];
+    double c[7];          // kind coefficients (gaussblur: f, s0, s1, s2, s4, s5, s8)
+};
+
+struct Item {
+    int grid, bx, by, bz;
+};
+
+struct BatchState {
+    unsigned long long step;   // steps completed by the batch
+    unsigned counter;          // next work item
+    unsigned done;             // workgroups finished this launch
+};
+
+// Not inlined: each variant then gets its own register allocation and the
+// kernel needs the maximum of them (up to 116), not their union (which spills
+// at the 128 registers a 1024-thread workgroup allows).
+// Tile staging memory of batch_k, sized for the largest variant.  At file
+// scope so that the non-inlined tile bodies address it as LDS.
+__shared__ __attribute__((aligned(16))) char batch_lds[BATCH_LDS];
+
+template <int K, int DT, int LPR>
+__device__ __forceinline__ void run_tile(const GridDesc* __restrict__ gp, int bx, int by,
+                                                   int bz, unsigned long long step)
+{
+    const GridDesc& g = *gp;
+    using C = VarCfg<K, DT, LPR>;
+    using Op = typename Kind<K>::template Op<C>;
+    using S = typename C::S;
+    using F = typename C::F;
+    using B = typename Kind<K>::B;
+    const long long ls = (long long)step - g.step0;
+    Op op{};
+    if constexpr (K == KG_KIND_WAVE13PT) {
+        op.side[0] = (const S*)g.lev[ls % 3];
+        op.src = (const S*)g.lev[(ls + 1) % 3];
+        op.dst = (S*)g.lev[(ls + 2) % 3];
+        op.m0 = (F)g.c[0];
+        op.m1 = (F)g.c[1];
+        op.m2 = (F)g.c[2];
+    } else if constexpr (K == KG_KIND_LAPLACIAN) {
+        op.src = (const S*)g.lev[ls & 1];
+        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.alpha = (F)g.c[0];
+        op.beta = (F)g.c[1];
+    } else {
+        op.src = (const S*)g.lev[ls & 1];
+        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.f = (F)g.c[0];
+        op.s0 = (F)g.c[1];
+        op.s1 = (F)g.c[2];
+        op.s2 = (F)g.c[3];
+        op.s4 = (F)g.c[4];
+        op.s5 = (F)g.c[5];
+        op.s8 = (F)g.c[6];
+    }
+    kgp::plane_tile<C, B::LO, B::HI, B::ZLO, B::ZHI, Op>(
+        (F(*)[C::ROWS][C::LW])batch_lds, g.nx, g.ny, g.ns, g.sy, g.sz, bx, by, bz, g.kc, op);
+}
+
+__global__ __launch_bounds__(64 * BW)
+void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items, int nitems,
+             BatchState* st)
+{
+    __shared__ int next;
+    const unsigned long long step = __hip_atomic_load(&st->step, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        if (threadIdx.x == 0) next = (int)atomicAdd(&st->counter, 1u);
+        __syncthreads();
+        const int i = next;
+        __syncthreads();
+        if (i >= nitems) break;
+        const Item it = items[i];
+        const GridDesc* g = grids + it.grid;
+        switch (g->var) {
+#define KG_VAR_CASE(K, DT, LI) \
+    case var_id(K, DT, LI): run_tile<K, DT, LPRS[LI]>(g, it.bx, it.by, it.bz, step); break;
+            KG_VAR_LIST(KG_VAR_CASE)
+#undef KG_VAR_CASE
+        default: break;
+        }
+        __syncthreads();   // the next tile restages LDS
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&st->done, 1u) == gridDim.x - 1) {
+            // every workgroup has taken its last item and read `step`
+            atomicExch(&st->counter, 0u);
+            atomicExch(&st->done, 0u);
+            atomicExch(&st->step, step + 1);
+            __threadfence();
+        }
+    }
+}
+
+__global__ void state_set_k(BatchState* st, unsigned long long step)
+{
+    if (threadIdx.x == 0) {
+        atomicExch(&st->step, step);
+        atomicExch(&st->counter, 0u);
+        atomicExch(&st->done, 0u);
+    }
+}
+
+// ------------------------------------------------------------ host side
+struct Arena {
+    // First-fit suballocator over hipMalloc'd chunks; blocks are 256-byte
+    // aligned.  free() coalesces neighbours; chunks are released at the end.
+    struct Chunk {
+        char* base;
+        size_t size;
+        std::map<size_t, size_t> free_;   // offset -> size
+    };
+    std::vector<Chunk> chunks;
+    size_t chunk_size = (size_t)1 << 30;
+    size_t reserved = 0;
+
+    static size_t up(size_t n) { return (n + 255) & ~(size_t)255; }
+
+    int alloc(size_t n, void** out)
+    {
+        n = up(n);
+        for (auto& c : chunks)
+            for (auto itf = c.free_.begin(); itf != c.free_.end(); ++itf)
+                if (itf->second >= n) {
+                    const size_t off = itf->first, sz = itf->second;
+                    c.free_.erase(itf);
+                    if (sz > n) c.free_[off + n] = sz - n;
+                    *out = c.base + off;
+                    return KG_OK;
+                }
+        const size_t cs = std::max(chunk_size, n);
+        char* p = nullptr;
+        hipError_t e = hipMalloc((void**)&p, cs);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return kg_set_error(KG_ERR_NOMEM, "kg_batch: device allocation of %zu bytes failed: %s",
+                                cs, hipGetErrorString(e));
+        }
+        reserved += cs;
+        chunks.push_back(Chunk{p, cs, {}});
+        if (cs > n) chunks.back().free_[n] = cs - n;
+        *out = p;
+        return KG_OK;
+    }
+    void release(void* ptr, size_t n)
+    {
+        n = up(n);
+        for (auto& c : chunks) {
+            char* p = (char*)ptr;
+            if (p < c.base || p >= c.base + c.size) continue;
+            size_t off = (size_t)(p - c.base);
+            auto nx = c.free_.lower_bound(off);
+            if (nx != c.free_.end() && off + n == nx->first) {
+                n += nx->second;
+                c.free_.erase(nx);
+            }
+            auto pv = c.free_.lower_bound(off);
+            if (pv != c.free_.begin()) {
+                --pv;
+                if (pv->first + pv->second == off) {
+                    off = pv->first;
+                    n += pv->second;
+                    c.free_.erase(pv);
+                }
+            }
+            c.free_[off] = n;
+            return;
+        }
+    }
+    void destroy()
+    {
+        for (auto& c : chunks) (void)hipFree(c.base);
+        chunks.clear();
+        reserved = 0;
+    }
+};
+
+struct HostGrid {
+    kg_grid_spec spec;
+    bool alive;
+    int li, kc;            // tile shape: LPRS[li], planes per tile
+    long long sy, sz;      // pitches (elements)
+    size_t level_bytes;
+    long long step0;
+    void* lev[3];
+};
+
+int nlev_of(int kind) { return kind == KG_KIND_WAVE13PT ? 3 : 2; }
+
+int size_class(const kg_grid_spec& s)
+{
+    auto bucket = [](int n) { return n <= 24 ? 0 : n <= 48 ? 1 : n <= 96 ? 2 : n <= 192 ? 3 : 4; };
+    return ((s.kind * NTYPE + s.dtype) * 5 + bucket(s.nx)) * 5 + bucket(s.ny);
+}
+
+void default_shape(const kg_grid_spec& s, int& li, int& kc)
+{
+    const int vec = s.dtype == KG_F64 ? 2 : 4;
+    li = 0;
+    if (s.nx <= 16 * vec) li = 2;
+    else if (s.nx <= 32 * vec) li = 1;
+    kc = s.ns <= 24 ? 8 : s.ns <= 48 ? 16 : 32;
+}
+
+}  // namespace
+
+struct kg_batch {
+    kg_ctx* ctx;
+    std::vector<HostGrid> grids;
+    Arena arena;
+    std::map<int, std::pair<int, int>> tuned;   // size class -> (li, kc)
+    GridDesc* d_desc = nullptr;
+    int desc_cap = 0;
+    Item* d_items = nullptr;
+    int item_cap = 0;
+    int nitems = 0;
+    BatchState* d_state = nullptr;
+    bool dirty = true;
+    long long steps = 0;                       // steps issued
+    int nwg = 0;                               // persistent workgroups
+    std::map<int, hipGraphExec_t> graphs;      // steps per replay -> graph
+    double pts_per_step = 0, bytes_per_step = 0;
+};
+
+namespace {
+
+void drop_graphs(kg_batch* b)
+{
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+// Tiles of grid g with shape (li, kc), appended to out.
+void cut_tiles(const HostGrid& g, int gi, int li, int kc, std::vector<Item>& out)
+{
+    const VarInfo& v = VARS[var_id(g.spec.kind, g.spec.dtype, li)];
+    const int nx = g.spec.nx, ny = g.spec.ny, ns = g.spec.ns;
+    const int nxi = nx - v.lo - v.hi, nyi = ny - v.lo - v.hi, nzi = ns - v.zlo - v.zhi;
+    if (nxi <= 0 || nyi <= 0 || nzi <= 0) return;
+    const int ntx = (nx + v.txw - 1) / v.txw;
+    const int nty = (nyi + v.ty - 1) / v.ty;
+    const int nkc = (nzi + kc - 1) / kc;
+    for (int bz = 0; bz < nkc; bz++)
+        for (int by = 0; by < nty; by++)
+            for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bz});
+}
+
+GridDesc make_desc(const HostGrid& g, int li, int kc)
+{
+    GridDesc d{};
+    d.var = var_id(g.spec.kind, g.spec.dtype, li);
+    d.nx = g.spec.nx;
+    d.ny = g.spec.ny;
+    d.ns = g.spec.ns;
+    d.kc = kc;
+    d.nlev = nlev_of(g.spec.kind);
+    d.sy = g.sy;
+    d.sz = g.sz;
+    d.step0 = g.step0;
+    for (int l = 0; l < 3; l++) d.lev[l] = g.lev[l];
+    const double* c = g.spec.coef;
+    if (g.spec.kind == KG_KIND_GAUSSBLUR3) {
+        // f = 1/(s0 + 4(s1+s2+s4+s8) + 8 s5) in the grid's arithmetic type
+        // (gaussblur/gaussblur.c:65)
+        if (g.spec.dtype == KG_F64) d.c[0] = 1. / (c[0] + 4 * (c[1] + c[2] + c[3] + c[5]) + 8 * c[4]);
+        else {
+            const float s0 = (float)c[0], s1 = (float)c[1], s2 = (float)c[2], s4 = (float)c[3],
+                        s5 = (float)c[4], s8 = (float)c[5];
+            d.c[0] = (double)(1.f / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5));
+        }
+        for (int q = 0; q < 6; q++) d.c[1 + q] = c[q];
+    } else {
+        for (int q = 0; q < 6; q++) d.c[q] = c[q];
+    }
+    return d;
+}
+
+int ensure_device_arrays(kg_batch* b, int ndesc, int nitems)
+{
+    if (ndesc > b->desc_cap) {
+        if (b->d_desc) KG_HIP(hipFree(b->d_desc));
+        b->d_desc = nullptr;
+        const int cap = std::max(ndesc, 2 * b->desc_cap);
+        KG_HIP(hipMalloc((void**)&b->d_desc, (size_t)cap * sizeof(GridDesc)));
+        b->desc_cap = cap;
+    }
+    if (nitems > b->item_cap) {
+        if (b->d_items) KG_HIP(hipFree(b->d_items));
+        b->d_items = nullptr;
+        const int cap = std::max(nitems, 2 * b->item_cap);
+        KG_HIP(hipMalloc((void**)&b->d_items, (size_t)cap * sizeof(Item)));
+        b->item_cap = cap;
+    }
+    return KG_OK;
+}
+
+// Shape of grid g: the tuned one of its size class, else the default.
+void shape_of(const kg_batch* b, const HostGrid& g, int& li, int& kc)
+{
+    auto it = b->tuned.find(size_class(g.spec));
+    if (it != b->tuned.end()) {
+        li = it->second.first;
+        kc = it->second.second;
+    } else {
+        default_shape(g.spec, li, kc);
+    }
+}
+
+// Upload descriptors and the work list (largest grids' tiles first).  With
+// only_class >= 0, only that size class is listed, with shape (li, kc).
+int build_work(kg_batch* b, int only_class = -1, int li_force = 0, int kc_force = 0)
+{
+    std::vector<GridDesc> descs(b->grids.size());
+    std::vector<int> order;
+    double pts = 0, bytes = 0;
+    for (size_t gi = 0; gi < b->grids.size(); gi++) {
+        HostGrid& g = b->grids[gi];
+        if (!g.alive) continue;
+        int li, kc;
+        if (only_class >= 0) {
+            if (size_class(g.spec) != only_class) continue;
+            li = li_force;
+            kc = kc_force;
+        } else {
+            shape_of(b, g, li, kc);
+        }
+        g.li = li;
+        g.kc = kc;
+        descs[gi] = make_desc(g, li, kc);
+        order.push_back((int)gi);
+        const double n = (double)g.spec.nx * g.spec.ny * g.spec.ns;
+        pts += n;
+        bytes += n * (g.spec.dtype == KG_F64 ? 8 : 4) * (g.spec.kind == KG_KIND_WAVE13PT ? 3 : 2);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        const kg_grid_spec &x = b->grids[a].spec, &y = b->grids[c].spec;
+        return (double)x.nx * x.ny * x.ns > (double)y.nx * y.ny * y.ns;
+    });
+    std::vector<Item> items;
+    for (int gi : order) cut_tiles(b->grids[gi], gi, b->grids[gi].li, b->grids[gi].kc, items);
+    if (int rc = ensure_device_arrays(b, (int)std::max<size_t>(descs.size(), 1),
+                                      (int)std::max<size_t>(items.size(), 1)))
+        return rc;
+    if (!descs.empty())
+        KG_HIP(hipMemcpyAsync(b->d_desc, descs.data(), descs.size() * sizeof(GridDesc),
+                              hipMemcpyHostToDevice, b->ctx->stream));
+    if (!items.empty())
+        KG_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(Item),
+                              hipMemcpyHostToDevice, b->ctx->stream));
+    KG_HIP(hipStreamSynchronize(b->ctx->stream));   // host vectors go out of scope
+    b->nitems = (int)items.size();
+    if (only_class < 0) {
+        b->pts_per_step = pts;
+        b->bytes_per_step = bytes;
+        b->dirty = false;
+    } else {
+        b->dirty = true;   // the full list must be rebuilt before the next real step
+    }
+    return KG_OK;
+}
+
+int launch_steps(kg_batch* b, int n)
+{
+    for (int s = 0; s < n; s++) {
+        hipLaunchKernelGGL(batch_k, dim3(b->nwg), dim3(64 * BW), 0, b->ctx->stream, b->d_desc,
+                           b->d_items, b->nitems, b->d_state);
+    }
+    KG_HIP(hipGetLastError());
+    return KG_OK;
+}
+
+int check_batch(kg_batch* b)
+{
+    if (!b) return kg_set_error(KG_ERR_INVALID, "kg_batch: null batch");
+    return kg_check_ctx(b->ctx);
+}
+
+int check_spec(const kg_grid_spec& s)
+{
+    if (s.kind < 0 || s.kind >= NKIND)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch: unknown kind %d", s.kind);
+    if (s.dtype != KG_F64 && s.dtype != KG_F32)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch: dtype %d not batched (F64 or F32)", s.dtype);
+    if (s.nx < 1 || s.ny < 1 || s.ns < 1 || (long long)s.nx * s.ny > (1LL << 31))
+        return kg_set_error(KG_ERR_INVALID, "kg_batch: bad grid %dx%dx%d", s.nx, s.ny, s.ns);
+    return KG_OK;
+}
+
+}  // namespace
+
+// ================================================================ C API
+
+extern "C" int kg_batch_create(kg_ctx* ctx, kg_batch** out)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (!out) return kg_set_error(KG_ERR_INVALID, "kg_batch_create: null argument");
+    *out = nullptr;
+    kg_batch* b = new (std::nothrow) kg_batch();
+    if (!b) return kg_set_error(KG_ERR_INVALID, "kg_batch_create: host allocation failed");
+    b->ctx = ctx;
+    KG_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc((void**)&b->d_state, sizeof(BatchState));
+    if (e != hipSuccess) {
+        delete b;
+        return kg_set_error(KG_ERR_NOMEM, "kg_batch_create: %s", hipGetErrorString(e));
+    }
+    hipLaunchKernelGGL(state_set_k, dim3(1), dim3(64), 0, ctx->stream, b->d_state, 0ull);
+    int per_cu = 0, cus = 0;
+    KG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_k, 64 * BW, 0));
+    KG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    b->nwg = std::max(1, per_cu) * std::max(1, cus);
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = b;
+    return KG_OK;
+}
+
+extern "C" int kg_batch_add(kg_batch* b, const kg_grid_spec* spec, int* id)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (!spec) return kg_set_error(KG_ERR_INVALID, "kg_batch_add: null spec");
+    if (int rc = check_spec(*spec)) return rc;
+    HostGrid g{};
+    g.spec = *spec;
+    g.alive = true;
+    const int vec = spec->dtype == KG_F64 ? 2 : 4;
+    const int es = spec->dtype == KG_F64 ? 8 : 4;
+    g.sy = (spec->nx + vec - 1) / vec * vec;
+    g.sz = g.sy * spec->ny;
+    g.level_bytes = (size_t)g.sz * spec->ns * es;
+    g.step0 = b->steps;
+    for (int l = 0; l < nlev_of(spec->kind); l++) {
+        if (int rc = b->arena.alloc(g.level_bytes, &g.lev[l])) {
+            for (int q = 0; q < l; q++) b->arena.release(g.lev[q], g.level_bytes);
+            return rc;
+        }
+    }
+    b->grids.push_back(g);
+    b->dirty = true;
+    drop_graphs(b);
+    if (id) *id = (int)b->grids.size() - 1;
+    return KG_OK;
+}
+
+extern "C" int kg_batch_remove(kg_batch* b, int id)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_remove: no grid %d", id);
+    // Work already queued on the stream may still read the levels.
+    KG_HIP(hipStreamSynchronize(b->ctx->stream));
+    HostGrid& g = b->grids[id];
+    for (int l = 0; l < nlev_of(g.spec.kind); l++) b->arena.release(g.lev[l], g.level_bytes);
+    g.alive = false;
+    b->dirty = true;
+    drop_graphs(b);
+    return KG_OK;
+}
+
+static int level_copy(kg_batch* b, int id, int level, void* host, bool up)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch: no grid %d", id);
+    HostGrid& g = b->grids[id];
+    if (level < 0 || level >= nlev_of(g.spec.kind))
+        return kg_set_error(KG_ERR_INVALID, "kg_batch: grid %d has no level %d", id, level);
+    if (!host) return kg_set_error(KG_ERR_INVALID, "kg_batch: null host buffer");
+    const size_t es = g.spec.dtype == KG_F64 ? 8 : 4;
+    const size_t w = (size_t)g.spec.nx * es, dp = (size_t)g.sy * es;
+    const size_t h = (size_t)g.spec.ny * g.spec.ns;
+    if (up)
+        KG_HIP(hipMemcpy2DAsync(g.lev[level], dp, host, w, w, h, hipMemcpyHostToDevice,
+                                b->ctx->stream));
+    else
+        KG_HIP(hipMemcpy2DAsync(host, w, g.lev[level], dp, w, h, hipMemcpyDeviceToHost,
+                                b->ctx->stream));
+    KG_HIP(hipStreamSynchronize(b->ctx->stream));
+    return KG_OK;
+}
+
+extern "C" int kg_batch_upload(kg_batch* b, int id, int level, const void* host)
+{
+    return level_copy(b, id, level, (void*)host, true);
+}
+
+extern "C" int kg_batch_download(kg_batch* b, int id, int level, void* host)
+{
+    return level_copy(b, id, level, host, false);
+}
+
+extern "C" int kg_batch_newest_level(kg_batch* b, int id, int* level)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive || !level)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_newest_level: bad grid %d", id);
+    const HostGrid& g = b->grids[id];
+    const long long ls = b->steps - g.step0;
+    // laplacian/gaussblur: step s writes level (s+1)%2; wave13pt: level (s+2)%3
+    *level = g.spec.kind == KG_KIND_WAVE13PT ? (int)((ls + 1) % 3) : (int)(ls % 2);
+    return KG_OK;
+}
+
+extern "C" int kg_batch_steps_done(kg_batch* b, int id, long long* steps)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive || !steps)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_steps_done: bad grid %d", id);
+    *steps = b->steps - b->grids[id].step0;
+    return KG_OK;
+}
+
+extern "C" int kg_batch_step(kg_batch* b, int nsteps, int use_graph)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (nsteps < 0) return kg_set_error(KG_ERR_INVALID, "kg_batch_step: negative step count");
+    if (nsteps == 0) return KG_OK;
+    if (b->dirty) {
+        if (int rc = build_work(b)) return rc;
+        drop_graphs(b);
+    }
+    kg_register_kernel("batch", (const void*)batch_k);
+    if (int rc = kg_launch_begin(b->ctx, "batch")) return rc;
+    if (b->nitems > 0) {
+        if (use_graph) {
+            auto it = b->graphs.find(nsteps);
+            if (it == b->graphs.end()) {
+                hipGraph_t graph;
+                KG_HIP(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+                int rc = launch_steps(b, nsteps);
+                hipError_t e = hipStreamEndCapture(b->ctx->stream, &graph);
+                if (rc) return rc;
+                KG_HIP(e);
+                hipGraphExec_t exec;
+                e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                KG_HIP(e);
+                it = b->graphs.emplace(nsteps, exec).first;
+            }
+            KG_HIP(hipGraphLaunch(it->second, b->ctx->stream));
+        } else if (int rc = launch_steps(b, nsteps)) {
+            return rc;
+        }
+    } else {
+        // nothing to compute: the step counter still advances
+        hipLaunchKernelGGL(state_set_k, dim3(1), dim3(64), 0, b->ctx->stream, b->d_state,
+                           (unsigned long long)(b->steps + nsteps));
+    }
+    b->steps += nsteps;
+    return kg_launch_end(b->ctx);
+}
+
+extern "C" int kg_batch_info(kg_batch* b, double* points_per_step, double* bytes_per_step,
+                             int* work_items, int* workgroups)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (b->dirty)
+        if (int rc = build_work(b)) return rc;
+    if (points_per_step) *points_per_step = b->pts_per_step;
+    if (bytes_per_step) *bytes_per_step = b->bytes_per_step;
+    if (work_items) *work_items = b->nitems;
+    if (workgroups) *workgroups = b->nwg;
+    return KG_OK;
+}
+
+extern "C" int kg_batch_tune(kg_batch* b, int reps, int* classes_tuned)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (reps < 1) reps = 1;
+    KG_HIP(hipStreamSynchronize(b->ctx->stream));
+    std::map<int, int> classes;
+    for (const auto& g : b->grids)
+        if (g.alive) classes[size_class(g.spec)]++;
+    // Tuning runs steps on a scratch step counter equal to the real one: each
+    // trial rewrites the level the next real step writes anyway.
+    BatchState* real = b->d_state;
+    BatchState* scratch = nullptr;
+    KG_HIP(hipMalloc((void**)&scratch, sizeof(BatchState)));
+    hipEvent_t e0, e1;
+    KG_HIP(hipEventCreate(&e0));
+    KG_HIP(hipEventCreate(&e1));
+    int rc = KG_OK;
+    int ntuned = 0;
+    for (const auto& cls : classes) {
+        const int c = cls.first;
+        float best = 1e30f;
+        std::pair<int, int> pick{-1, -1};
+        for (int li = 0; li < NLPR && !rc; li++)
+            for (int ki = 0; ki < NKC && !rc; ki++) {
+                if ((rc = build_work(b, c, li, KCS[ki]))) break;
+                b->d_state = scratch;
+                hipLaunchKernelGGL(state_set_k, dim3(1), dim3(64), 0, b->ctx->stream, scratch,
+                                   (unsigned long long)b->steps);
+                float ms = 0;
+                rc = launch_steps(b, 1);   // warm-up
+                if (!rc) {
+                    hipLaunchKernelGGL(state_set_k, dim3(1), dim3(64), 0, b->ctx->stream, scratch,
+                                       (unsigned long long)b->steps);
+                    (void)hipEventRecord(e0, b->ctx->stream);
+                    for (int r = 0; r < reps && !rc; r++) {
+                        hipLaunchKernelGGL(state_set_k, dim3(1), dim3(64), 0, b->ctx->stream,
+                                           scratch, (unsigned long long)b->steps);
+                        rc = launch_steps(b, 1);
+                    }
+                    (void)hipEventRecord(e1, b->ctx->stream);
+                    if (hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                        rc = kg_set_error(KG_ERR_HIP, "kg_batch_tune: event timing failed");
+                }
+                b->d_state = real;
+                if (!rc && ms < best) {
+                    best = ms;
+                    pick = {li, KCS[ki]};
+                }
+            }
+        if (rc) break;
+        b->tuned[c] = pick;
+        ntuned++;
+    }
+    b->d_state = real;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    b->dirty = true;
+    drop_graphs(b);
+    if (classes_tuned) *classes_tuned = ntuned;
+    return rc;
+}
+
+extern "C" int kg_batch_grid_shape(kg_batch* b, int id, int* lanes_per_row, int* planes_per_tile)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_grid_shape: no grid %d", id);
+    if (b->dirty)
+        if (int rc = build_work(b)) return rc;
+    if (lanes_per_row) *lanes_per_row = LPRS[b->grids[id].li];
+    if (planes_per_tile) *planes_per_tile = b->grids[id].kc;
+    return KG_OK;
+}
+
+extern "C" int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int planes_per_tile)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: no grid %d", id);
+    int li = -1;
+    for (int q = 0; q < NLPR; q++)
+        if (LPRS[q] == lanes_per_row) li = q;
+    if (li < 0 || planes_per_tile < 1)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: lanes per row must be 16, 32 "
+                                            "or 64 and planes per tile positive");
+    b->tuned[size_class(b->grids[id].spec)] = {li, planes_per_tile};
+    b->dirty = true;
+    drop_graphs(b);
+    return KG_OK;
+}
+
+extern "C" int kg_batch_memory(kg_batch* b, size_t* reserved, size_t* in_use)
+{
+    if (int rc = check_batch(b)) return rc;
+    size_t used = 0;
+    for (const auto& g : b->grids)
+        if (g.alive) used += Arena::up(g.level_bytes) * nlev_of(g.spec.kind);
+    if (reserved) *reserved = b->arena.reserved;
+    if (in_use) *in_use = used;
+    return KG_OK;
+}
+
+extern "C" int kg_batch_destroy(kg_batch* b)
+{
+    if (!b) return KG_OK;
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graphs(b);
+    b->arena.destroy();
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_state) (void)hipFree(b->d_state);
+    delete b;
+    return KG_OK;
+}
+
+int kg_preload_batch() { return kg_preload_kernel((const void*)batch_k); }

@@ -17,7 +17,11 @@ struct kg_ctx {
     long long launches;               // launches recorded so far
     long long drained;                // launches already returned by kg_timing_drain
     const char* last_kernel;          // name of the most recent launch
+    double* stats;                    // device partials of kg_field_stats
 };
+
+// Bytes of the context's kg_field_stats scratch.
+size_t kg_stats_scratch_bytes();
 
 // Records the most recent error message (thread-local) and returns the code.
 int kg_set_error(int code, const char* fmt, ...);
@@ -52,6 +56,9 @@ int kg_preload_tricubic();
 int kg_preload_stencil2d();
 int kg_preload_matmul();
 int kg_preload_util();
+int kg_preload_life();
+int kg_preload_typed();
+int kg_preload_batch();
 int kg_preload_kernel(const void* func);
 
 // Validation helpers shared by the entry points.

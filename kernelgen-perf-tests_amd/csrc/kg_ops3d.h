@@ -1,0 +1,120 @@
+// kg_ops3d.h -- stencil bodies on the plane engine (kg_plane.h) that exist in
+// several storage types: the FP64 drop-in kernels, the FP32/BF16 precision
+// variants and the batched/served grids all run these same bodies.  Each op
+// computes the VEC points of one lane at plane k (see kgp::View) in the
+// reference's source order, in the arithmetic type C::F, and stores them in
+// the storage type C::S.
+#pragma once
+
+#include "kg_plane.h"
+
+namespace kgops {
+
+// laplacian/laplacian.c:93-97 (interior [1,n-1)^3)
+template <class C>
+struct LaplacianOp {
+    using S = typename C::S;
+    using F = typename C::F;
+    const S* src;
+    const S* side[1];
+    S* dst;
+    F alpha, beta;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        F r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = alpha * p.col(v, 0) + beta * (
+                p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
+                p.col(v, 1) + p.col(v, -1));
+        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
+// wave13pt/wave13pt.c:640-651 (interior [2,n-2)^3); main field w1, side
+// stream w0, output w2
+template <class C>
+struct Wave13ptOp {
+    using S = typename C::S;
+    using F = typename C::F;
+    const S* src;
+    const S* side[1];
+    S* dst;
+    F m0, m1, m2;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        F r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = m0 * p.col(v, 0) - p.side(0, v) +
+                m1 * (
+                    p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                    p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
+                    p.col(v, 1) + p.col(v, -1)) +
+                m2 * (
+                    p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
+                    p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
+                    p.col(v, 2) + p.col(v, -2));
+        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
+// gaussblur/gaussblur.c:85-92 applied to every xy-plane of a 3D grid:
+// xy-interior [2,n-2), all planes.  f = 1/(s0 + 4(s1+s2+s4+s8) + 8 s5)
+// (gaussblur/gaussblur.c:65) is computed by the caller.
+template <class C>
+struct Gaussblur3Op {
+    using S = typename C::S;
+    using F = typename C::F;
+    const S* src;
+    const S* side[1];
+    S* dst;
+    F f, s0, s1, s2, s4, s5, s8;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        F r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) {
+#define W(di, dj) p.at(v, di, dj, 0)
+            r[v] = f * (
+                s0 * W(0, 0) +
+                s1 * (W(-1, 0) + W(1, 0) + W(0, -1) + W(0, 1)) +
+                s2 * (W(-1, -1) + W(1, -1) + W(-1, 1) + W(1, 1)) +
+                s4 * (W(-2, 0) + W(2, 0) + W(0, -2) + W(0, 2)) +
+                s5 * (W(-2, -1) + W(-1, -2) + W(1, -2) + W(2, -1) +
+                      W(-2, 1) + W(-1, 2) + W(1, 2) + W(2, 1)) +
+                s8 * (W(-2, -2) + W(2, -2) + W(-2, 2) + W(2, 2)));
+#undef W
+        }
+        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
+// Interior bounds of each op: xy [LO, n-HI), z [ZLO, ns-ZHI).
+template <template <class> class Op> struct Bounds;
+template <> struct Bounds<LaplacianOp> { static constexpr int LO = 1, HI = 1, ZLO = 1, ZHI = 1; };
+template <> struct Bounds<Wave13ptOp> { static constexpr int LO = 2, HI = 2, ZLO = 2, ZHI = 2; };
+template <> struct Bounds<Gaussblur3Op> { static constexpr int LO = 2, HI = 2, ZLO = 0, ZHI = 0; };
+
+}  // namespace kgops
+
+namespace kgops {
+
+// Engine shape each op needs: halo width, LDS planes, register column and
+// side streams (kgp::Cfg parameters H, LL, LR, ZL, ZR, NAUX).
+template <template <class> class Op> struct Shape;
+template <> struct Shape<LaplacianOp> { static constexpr int H = 1, LL = 0, LR = 0, ZL = 1, ZR = 1, NAUX = 0; };
+template <> struct Shape<Wave13ptOp> { static constexpr int H = 2, LL = 0, LR = 0, ZL = 2, ZR = 2, NAUX = 1; };
+template <> struct Shape<Gaussblur3Op> { static constexpr int H = 2, LL = 0, LR = 0, ZL = 0, ZR = 0, NAUX = 0; };
+
+// Engine configuration of op Op with storage type T, VEC points per lane, W
+// waves, prefetch depth D, KC planes per single-grid tile, LPR lanes per row.
+template <template <class> class Op, class T, int VEC, int W, int D, int KC, int LPR = 64>
+using OpCfg = kgp::Cfg<VEC, W, Shape<Op>::H, Shape<Op>::LL, Shape<Op>::LR, Shape<Op>::ZL,
+                       Shape<Op>::ZR, D, Shape<Op>::NAUX, KC, T, LPR>;
+
+}  // namespace kgops

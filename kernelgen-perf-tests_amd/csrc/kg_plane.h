@@ -1,23 +1,32 @@
 // kg_plane.h -- plane-staged z-marching engine for the 3D stencils.
 //
-// Workgroup = W wave64s.  Wave w owns one row of a tile that is 64*VEC points
-// wide (VEC = 2: each lane holds two consecutive x points, loaded with one
-// 16-byte access) and W rows tall; the H rows at either end are y-halo rows
-// that are loaded and staged but not computed, so a tile computes W-2H rows.
-// The workgroup marches its tile through a chunk of z-planes:
+// Workgroup = W wave64s.  A wave covers RPW = 64/LPR rows of LPR lanes; each
+// lane holds VEC consecutive x points of its row, loaded with one access of
+// VEC * sizeof(storage) bytes (16 bytes for FP64 VEC=2, FP32 VEC=4).  A tile
+// is LPR*VEC points wide and W*RPW rows tall; the H rows at either end are
+// y-halo rows that are loaded and staged but not computed.  LPR = 64 (one row
+// per wave) is the shape for wide grids; LPR = 32 / 16 fold two / four short
+// rows into a wave so that narrow grids keep their lanes busy.  The workgroup
+// marches its tile through a chunk of z-planes:
 //
 //   * every global load of the main field is issued D planes ahead of use
-//     into a register ring (own row only, plus the H x-halo columns on lanes
-//     0..H-1 and 64-H..63), so the only vector-memory traffic in the loop is
-//     this prefetch stream plus the side streams and the output stores --
-//     neighbour values never wait behind an HBM miss;
+//     into a register ring (own row only, plus the H x-halo columns on the
+//     first and last H lanes of each row), so the only vector-memory traffic
+//     in the loop is this prefetch stream plus the side streams and the output
+//     stores -- neighbour values never wait behind an HBM miss;
 //   * planes k-LL .. k+LR of the main field are staged in LDS (LL+LR+2 plane
 //     slots, one barrier per plane) and give every xy-neighbour;
 //   * the register ring keeps the own-row column k-ZL .. k+ZR for z-neighbours;
 //   * up to NAUX side streams (arrays read only at the point itself) ride in
 //     a second prefetch ring of depth D.
 //
-// Measured on MI355X at 512x256x256 FP64 (cold caches): laplacian 105 us with
+// Storage and arithmetic types come from a type pair T (F64: double/double,
+// F32: float/float, BF16: bfloat16 storage with float arithmetic); values are
+// converted at load and rounded to nearest-even at store.  Rows are `sy`
+// elements apart and planes `sz` apart (sy = nx, sz = nx*ny for the dense
+// layout of the C ABI; batched grids pad sy to a multiple of VEC).
+//
+// Measured on MI355X at 512x256x256 FP64 (cold caches): laplacian 102 us with
 // this engine against 143 us for per-thread z-marching with cache-served
 // neighbours; HBM reads 1.06x the input size (the y-halo rows are L2 hits).
 #pragma once
@@ -26,12 +35,72 @@
 
 namespace kgp {
 
-typedef double d2 __attribute__((ext_vector_type(2)));
+// ---------------------------------------------------------------- types
+struct F64 {
+    using S = double;
+    using F = double;
+};
+struct F32 {
+    using S = float;
+    using F = float;
+};
+struct BF16 {
+    using S = uint16_t;   // bfloat16 bit pattern
+    using F = float;
+};
 
-template <int VEC_, int W_, int H_, int LL_, int LR_, int ZL_, int ZR_, int D_, int NAUX_, int KC_>
+__device__ __forceinline__ float bf16_in(uint16_t s) { return __uint_as_float((unsigned)s << 16); }
+
+// float -> bfloat16, round to nearest even; NaNs stay (quiet) NaNs.
+__device__ __forceinline__ uint16_t bf16_out(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+template <class T>
+__device__ __forceinline__ typename T::F ld_cvt(typename T::S s)
+{
+    if constexpr (sizeof(typename T::S) == 2) return bf16_in(s);
+    else return s;
+}
+template <class T>
+__device__ __forceinline__ typename T::S st_cvt(typename T::F f)
+{
+    if constexpr (sizeof(typename T::S) == 2) return bf16_out(f);
+    else return f;
+}
+
+// An unsigned vector of BYTES bytes (one global access).
+template <int BYTES> struct Raw;
+template <> struct Raw<16> { typedef unsigned t __attribute__((ext_vector_type(4))); };
+template <> struct Raw<8> { typedef unsigned t __attribute__((ext_vector_type(2))); };
+template <> struct Raw<4> { typedef unsigned t; };
+template <> struct Raw<2> { typedef unsigned short t; };
+
+// VEC consecutive storage values at p (aligned to the access size).
+template <class T, int VEC>
+__device__ __forceinline__ void vload(const typename T::S* p, typename T::F (&out)[VEC])
+{
+    using S = typename T::S;
+    using R = typename Raw<VEC * (int)sizeof(S)>::t;
+    const R r = *(const R*)p;
+    S s[VEC];
+    __builtin_memcpy(s, &r, sizeof r);
+#pragma unroll
+    for (int v = 0; v < VEC; v++) out[v] = ld_cvt<T>(s[v]);
+}
+
+// ---------------------------------------------------------------- config
+template <int VEC_, int W_, int H_, int LL_, int LR_, int ZL_, int ZR_, int D_, int NAUX_, int KC_,
+          class T_ = F64, int LPR_ = 64>
 struct Cfg {
-    static constexpr int VEC = VEC_;   // points per lane along x (1 or 2)
-    static constexpr int W = W_;       // waves (= tile rows, halo rows included)
+    using T = T_;
+    using S = typename T::S;           // storage type
+    using F = typename T::F;           // arithmetic type
+    static constexpr int VEC = VEC_;   // points per lane along x
+    static constexpr int W = W_;       // waves per workgroup
     static constexpr int H = H_;       // x and y halo width
     static constexpr int LL = LL_;     // LDS planes below k
     static constexpr int LR = LR_;     // LDS planes above k
@@ -39,26 +108,33 @@ struct Cfg {
     static constexpr int ZR = ZR_;     // register column above k
     static constexpr int D = D_;       // prefetch distance (planes)
     static constexpr int NAUX = NAUX_; // side streams read at the point only
-    static constexpr int KC = KC_;     // planes per workgroup
-    static constexpr int TXW = 64 * VEC;
+    static constexpr int KC = KC_;     // planes per workgroup (single-grid launches)
+    static constexpr int LPR = LPR_;   // lanes per row (64, 32 or 16)
+    static constexpr int RPW = 64 / LPR;             // rows per wave
+    static constexpr int ROWS = W * RPW;             // tile rows, halo rows included
+    static constexpr int TXW = LPR * VEC;            // tile width in points
     static constexpr int LW = TXW + 2 * H;
     static constexpr int NLS = LL + LR + 2;          // LDS plane slots
     static constexpr int ZLO = ZL;                   // ring starts at k-ZL
     static constexpr int ZHI = (ZR > LR ? ZR : LR);  // ring reaches k+ZHI before prefetch
     static constexpr int RS = ZLO + ZHI + D;         // ring slots: planes k-ZL .. k+ZHI+D-1
-    static constexpr int TY = W - 2 * H;             // computed rows per tile
+    static constexpr int TY = ROWS - 2 * H;          // computed rows per tile
     static constexpr int NA = NAUX > 0 ? NAUX : 1;
+    static constexpr size_t LDS_BYTES = sizeof(F) * NLS * ROWS * LW;
+    static_assert(LPR == 64 || LPR == 32 || LPR == 16, "lanes per row");
+    static_assert(LPR > 2 * H, "halo lanes of a row must not overlap");
+    static_assert(TY > 0, "tile has no computed rows");
 };
 
-template <int VEC, int H>
+template <class F, int VEC>
 struct Slot {
-    double v[VEC];
-    double h;   // x-halo value carried by lanes 0..H-1 and 64-H..63
+    F v[VEC];
+    F h;   // x-halo value carried by the first and last H lanes of a row
 };
 
-template <int VEC, int NA>
+template <class F, int VEC, int NA>
 struct ASlot {
-    double a[NA][VEC];
+    F a[NA][VEC];
 };
 
 struct Grid {
@@ -67,120 +143,124 @@ struct Grid {
     bool empty;
 };
 
-// Interior [lo, n-hi) along every axis.
+// Tiles of a grid whose xy-interior is [lo, n-hi) and z-interior [zlo, ns-zhi).
 template <class C>
-static inline Grid make_grid(int nx, int ny, int ns, int lo, int hi)
+static inline Grid make_grid(int nx, int ny, int ns, int lo, int hi, int zlo, int zhi, int kc)
 {
     Grid g;
-    const int nzi = ns - hi - lo, nyi = ny - hi - lo, nxi = nx - hi - lo;
+    const int nzi = ns - zhi - zlo, nyi = ny - hi - lo, nxi = nx - hi - lo;
     g.empty = nxi <= 0 || nyi <= 0 || nzi <= 0;
     g.ntx = (nx + C::TXW - 1) / C::TXW;
     g.nty = g.empty ? 0 : (nyi + C::TY - 1) / C::TY;
-    g.nkc = g.empty ? 0 : (nzi + C::KC - 1) / C::KC;
+    g.nkc = g.empty ? 0 : (nzi + kc - 1) / kc;
     g.blocks = (unsigned)g.ntx * (unsigned)g.nty * (unsigned)g.nkc;
     return g;
+}
+template <class C>
+static inline Grid make_grid(int nx, int ny, int ns, int lo, int hi)
+{
+    return make_grid<C>(nx, ny, ns, lo, hi, lo, hi, C::KC);
 }
 
 // Per-lane view handed to a stencil's compute function.
 template <class C>
 struct View {
-    const double (*lds)[C::W][C::LW];   // [NLS][W][LW]
-    const Slot<C::VEC, C::H>* ring;     // ring[ZLO + dz] = own row at plane k+dz
-    const ASlot<C::VEC, C::NA>* aux;    // aux[0] = side streams at plane k
-    int k, w, lane;
+    using F = typename C::F;
+    const F (*lds)[C::ROWS][C::LW];            // [NLS][ROWS][LW]
+    const Slot<F, C::VEC>* ring;               // ring[ZLO + dz] = own row at plane k+dz
+    const ASlot<F, C::VEC, C::NA>* aux;        // aux[0] = side streams at plane k
+    int k, row, cl;   // plane, tile row and lane-in-row of this lane
     // main field at (x + v + dx, j + dy, k + dz) from LDS, |dx|,|dy| <= H,
     // -LL <= dz <= LR
-    __device__ __forceinline__ double at(int v, int dx, int dy, int dz) const
+    __device__ __forceinline__ F at(int v, int dx, int dy, int dz) const
     {
         const int s = (k + dz) % C::NLS;
-        return lds[s][w + dy][C::H + lane * C::VEC + v + dx];
+        return lds[s][row + dy][C::H + cl * C::VEC + v + dx];
     }
     // main field at (x + v, j, k + dz) from registers, -ZL <= dz <= ZHI
-    __device__ __forceinline__ double col(int v, int dz) const { return ring[C::ZLO + dz].v[v]; }
-    __device__ __forceinline__ double side(int a, int v) const { return aux[0].a[a][v]; }
+    __device__ __forceinline__ F col_(int v, int dz) const { return ring[C::ZLO + dz].v[v]; }
+    __device__ __forceinline__ F col(int v, int dz) const { return col_(v, dz); }
+    __device__ __forceinline__ F side(int a, int v) const { return aux[0].a[a][v]; }
 };
 
-// kernel: Op supplies
-//   const double* src;                  main field
-//   const double* side[NAUX];           side streams (may be unused)
-//   __device__ void point(const View<C>&, int v, long long o, double& out...)
-// Op::run(view, ob, x, xmask) computes the VEC points of this lane at plane
-// k, row j (ob = k*sz + j*nx) and stores them; xmask[v] says whether point
-// x+v is inside the interior.
-template <class C, int LO, int HI, class Op>
-__global__ __launch_bounds__(64 * C::W)
-void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
+// One tile (bx, by, bz) of a grid: xy-interior [LO, n-HI), z-interior
+// [ZB0, ns-ZB1), planes ZB0 + bz*kc .. +kc.  Every thread of the workgroup
+// calls it (it contains barriers).  Op supplies
+//   const S* src;            main field
+//   const S* side[NAUX];     side streams (may be unused)
+//   void run(const View<C>&, long long ob, int x, const bool (&xmask)[VEC]) const;
+// run computes the VEC points of this lane at plane k, row j (ob = k*sz +
+// j*sy, x the first point's column) and stores them; xmask[v] says whether
+// point x+v is inside the interior.
+template <class C, int LO, int HI, int ZB0, int ZB1, class Op>
+__device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW], int nx, int ny,
+                                           int ns, long long sy, long long sz, int bx, int by,
+                                           int bz, int kc, const Op& op)
 {
-    __shared__ double lds[C::NLS][C::W][C::LW];
+    using S = typename C::S;
+    using F = typename C::F;
+    using T = typename C::T;
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
-    const int bx = (int)(b % (unsigned)ntx);
-    b /= (unsigned)ntx;
-    const int by = (int)(b % (unsigned)nty);
-    const int bz = (int)(b / (unsigned)nty);
+    const int c = lane % C::LPR;
+    const int row = w * C::RPW + lane / C::LPR;
     const int x0 = bx * C::TXW;
     const int y0 = LO + by * C::TY;
-    const int k0 = LO + bz * C::KC;
-    const int k1 = min(k0 + C::KC, ns - HI);
-    const int j = y0 - C::H + w;
-    const int x = x0 + lane * C::VEC;
-    const long long sz = (long long)nx * ny;
+    const int k0 = ZB0 + bz * kc;
+    const int k1 = min(k0 + kc, ns - ZB1);
+    const int j = y0 - C::H + row;
+    const int x = x0 + c * C::VEC;
     const bool row_ok = j >= 0 && j < ny;
-    const bool comp_row = w >= C::H && w < C::W - C::H && j >= LO && j < ny - HI;
+    const bool comp_row = row >= C::H && row < C::ROWS - C::H && j >= LO && j < ny - HI;
     // x-halo column of this lane (if any)
     int hx = -1;
-    if (lane < C::H) hx = x0 - C::H + lane;
-    else if (lane >= 64 - C::H) hx = x0 + C::TXW + (lane - (64 - C::H));
+    if (c < C::H) hx = x0 - C::H + c;
+    else if (c >= C::LPR - C::H) hx = x0 + C::TXW + (c - (C::LPR - C::H));
     const bool h_ok = row_ok && hx >= 0 && hx < nx;
     bool p_ok[C::VEC];
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) p_ok[v] = row_ok && x + v < nx;
-    const bool vec_ok = C::VEC == 2 && p_ok[0] && p_ok[C::VEC - 1];
-    const double* src = op.src + (long long)j * nx;
+    const bool vec_ok = C::VEC > 1 && p_ok[C::VEC - 1];
+    const S* src = op.src + (long long)j * sy;
 
-    auto load = [&](int k, Slot<C::VEC, C::H>& s) {
+    auto load = [&](int k, Slot<F, C::VEC>& s) {
         if (k < 0 || k >= ns) return;
-        const double* pp = src + (long long)k * sz;
-        if (C::VEC == 2 && vec_ok) {
-            const d2 t = *(const d2*)(pp + x);
-            s.v[0] = t.x;
-            s.v[C::VEC - 1] = t.y;
+        const S* pp = src + (long long)k * sz;
+        if (C::VEC > 1 && vec_ok) {
+            vload<T, C::VEC>(pp + x, s.v);
         } else {
 #pragma unroll
-            for (int v = 0; v < C::VEC; v++) s.v[v] = p_ok[v] ? pp[x + v] : 0.0;
+            for (int v = 0; v < C::VEC; v++) s.v[v] = p_ok[v] ? ld_cvt<T>(pp[x + v]) : F(0);
         }
-        s.h = h_ok ? pp[hx] : 0.0;
+        s.h = h_ok ? ld_cvt<T>(pp[hx]) : F(0);
     };
-    auto load_aux = [&](int k, ASlot<C::VEC, C::NA>& s) {
+    auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s) {
         if (C::NAUX == 0 || !comp_row || k >= ns) return;
 #pragma unroll
         for (int a = 0; a < C::NAUX; a++) {
-            const double* pp = op.side[a] + (long long)j * nx + (long long)k * sz;
-            if (C::VEC == 2 && vec_ok) {
-                const d2 t = *(const d2*)(pp + x);
-                s.a[a][0] = t.x;
-                s.a[a][C::VEC - 1] = t.y;
+            const S* pp = op.side[a] + (long long)j * sy + (long long)k * sz;
+            if (C::VEC > 1 && vec_ok) {
+                vload<T, C::VEC>(pp + x, s.a[a]);
             } else {
 #pragma unroll
-                for (int v = 0; v < C::VEC; v++) s.a[a][v] = p_ok[v] ? pp[x + v] : 0.0;
+                for (int v = 0; v < C::VEC; v++) s.a[a][v] = p_ok[v] ? ld_cvt<T>(pp[x + v]) : F(0);
             }
         }
     };
-    auto stage = [&](int k, const Slot<C::VEC, C::H>& s) {
-        double(*L)[C::LW] = lds[k % C::NLS];
+    auto stage = [&](int k, const Slot<F, C::VEC>& s) {
+        F(*L)[C::LW] = lds[k % C::NLS];
 #pragma unroll
-        for (int v = 0; v < C::VEC; v++) L[w][C::H + lane * C::VEC + v] = s.v[v];
-        if (lane < C::H) L[w][lane] = s.h;
-        else if (lane >= 64 - C::H) L[w][C::H + C::TXW + (lane - (64 - C::H))] = s.h;
+        for (int v = 0; v < C::VEC; v++) L[row][C::H + c * C::VEC + v] = s.v[v];
+        if (c < C::H) L[row][c] = s.h;
+        else if (c >= C::LPR - C::H) L[row][C::H + C::TXW + (c - (C::LPR - C::H))] = s.h;
     };
 
-    Slot<C::VEC, C::H> ring[C::RS];
-    ASlot<C::VEC, C::NA> aux[C::D];
+    Slot<F, C::VEC> ring[C::RS];
+    ASlot<F, C::VEC, C::NA> aux[C::D];
 #pragma unroll
     for (int r = 0; r < C::RS; r++) {
 #pragma unroll
-        for (int v = 0; v < C::VEC; v++) ring[r].v[v] = 0.0;
-        ring[r].h = 0.0;
+        for (int v = 0; v < C::VEC; v++) ring[r].v[v] = F(0);
+        ring[r].h = F(0);
     }
 #pragma unroll
     for (int r = 0; r < C::RS; r++) load(k0 - C::ZLO + r, ring[r]);
@@ -192,10 +272,10 @@ void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
         if (dz >= -C::ZLO) {
             stage(k0 + dz, ring[C::ZLO + dz]);
         } else {
-            Slot<C::VEC, C::H> t;
-            t.h = 0.0;
+            Slot<F, C::VEC> t;
+            t.h = F(0);
 #pragma unroll
-            for (int v = 0; v < C::VEC; v++) t.v[v] = 0.0;
+            for (int v = 0; v < C::VEC; v++) t.v[v] = F(0);
             load(k0 + dz, t);
             stage(k0 + dz, t);
         }
@@ -206,18 +286,18 @@ void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
 
     for (int k = k0; k < k1; k++) {
         stage(k + C::LR, ring[C::ZLO + C::LR]);
-        Slot<C::VEC, C::H> nq;
-        nq.h = 0.0;
+        Slot<F, C::VEC> nq;
+        nq.h = F(0);
 #pragma unroll
-        for (int v = 0; v < C::VEC; v++) nq.v[v] = 0.0;
-        ASlot<C::VEC, C::NA> na;
+        for (int v = 0; v < C::VEC; v++) nq.v[v] = F(0);
+        ASlot<F, C::VEC, C::NA> na;
         const int kp = k + C::ZHI + C::D;    // plane entering the ring
         if (kp <= k1 - 1 + C::ZHI) load(kp, nq);
         if (k + C::D < k1) load_aux(k + C::D, na);
         __syncthreads();
         if (comp_row) {
-            View<C> vw{(const double(*)[C::W][C::LW])lds, ring, aux, k, w, lane};
-            op.run(vw, (long long)k * sz + (long long)j * nx, x, xmask);
+            View<C> vw{(const F(*)[C::ROWS][C::LW])lds, ring, aux, k, row, c};
+            op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
 #pragma unroll
         for (int r = 0; r < C::RS - 1; r++) ring[r] = ring[r + 1];
@@ -228,22 +308,52 @@ void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
     }
 }
 
-// Store the VEC results of a lane: one 16-byte non-temporal store when both
-// points are interior, else per point.
+// Single-grid launch in the dense layout: one workgroup per tile, tiles
+// enumerated x fastest, then y, then z-chunk, through the XCD swizzle.
+// z-interior [ZB0, ns-ZB1), by default the same as the xy-interior.
+template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI>
+__global__ __launch_bounds__(64 * C::W)
+void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
+{
+    __shared__ typename C::F lds[C::NLS][C::ROWS][C::LW];
+    unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
+    const int bx = (int)(b % (unsigned)ntx);
+    b /= (unsigned)ntx;
+    const int by = (int)(b % (unsigned)nty);
+    const int bz = (int)(b / (unsigned)nty);
+    plane_tile<C, LO, HI, ZB0, ZB1, Op>(lds, nx, ny, ns, nx, (long long)nx * ny, bx, by, bz,
+                                        C::KC, op);
+}
+
+// Store the VEC results of a lane: one vector non-temporal store when all
+// points are interior, else per point.  Values are rounded to the storage type.
+template <class T, int VEC>
+__device__ __forceinline__ void store(typename T::S* dst, long long ob, int x,
+                                      const typename T::F (&r)[VEC], const bool (&xmask)[VEC])
+{
+    using S = typename T::S;
+    bool all = true;
+#pragma unroll
+    for (int v = 0; v < VEC; v++) all = all && xmask[v];
+    if (VEC > 1 && all) {
+        using R = typename Raw<VEC * (int)sizeof(S)>::t;
+        S s[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) s[v] = st_cvt<T>(r[v]);
+        R raw;
+        __builtin_memcpy(&raw, s, sizeof raw);
+        kg_st((R*)(dst + ob + x), raw);
+    } else {
+#pragma unroll
+        for (int v = 0; v < VEC; v++)
+            if (xmask[v]) kg_st(dst + ob + x + v, st_cvt<T>(r[v]));
+    }
+}
 template <int VEC>
 __device__ __forceinline__ void store(double* dst, long long ob, int x, const double (&r)[VEC],
                                       const bool (&xmask)[VEC])
 {
-    if (VEC == 2 && xmask[0] && xmask[VEC - 1]) {
-        d2 t;
-        t.x = r[0];
-        t.y = r[VEC - 1];
-        kg_st((d2*)(dst + ob + x), t);
-    } else {
-#pragma unroll
-        for (int v = 0; v < VEC; v++)
-            if (xmask[v]) kg_st(dst + ob + x + v, r[v]);
-    }
+    store<F64, VEC>(dst, ob, x, r, xmask);
 }
 
 }  // namespace kgp

@@ -131,8 +131,16 @@ extern "C" int kg_init(kg_ctx** out, int device)
         return kg_set_error(KG_ERR_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(se));
     }
     c->last_kernel = "";
+    // FP64 partials of kg_field_stats (a few KB, owned by the context)
+    if (hipMalloc((void**)&c->stats, kg_stats_scratch_bytes()) != hipSuccess) {
+        (void)hipGetLastError();
+        c->stats = nullptr;
+        kg_destroy(c);
+        return kg_set_error(KG_ERR_NOMEM, "kg_init: statistics scratch allocation failed");
+    }
     int (*preload[])() = {kg_preload_stencil3d, kg_preload_tricubic, kg_preload_stencil2d,
-                          kg_preload_matmul, kg_preload_util};
+                          kg_preload_matmul, kg_preload_util, kg_preload_life,
+                          kg_preload_typed, kg_preload_batch};
     for (auto f : preload) {
         if (int rc = f()) {
             kg_destroy(c);
@@ -159,6 +167,7 @@ extern "C" int kg_destroy(kg_ctx* ctx)
         if (ctx->ev_start[i]) (void)hipEventDestroy(ctx->ev_start[i]);
         if (ctx->ev_stop[i]) (void)hipEventDestroy(ctx->ev_stop[i]);
     }
+    if (ctx->stats) (void)hipFree(ctx->stats);
     (void)hipStreamDestroy(ctx->stream);
     free(ctx->ev_start);
     free(ctx->ev_stop);

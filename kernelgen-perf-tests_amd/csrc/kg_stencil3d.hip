@@ -9,12 +9,15 @@
 // Arithmetic is written in the reference's source order and the library is
 // built with -ffp-contract=off: results are bit-identical to an IEEE build of
 // the reference kernels.
-#include "kg_plane.h"
+#include "kg_ops3d.h"
 #include "kg_zmarch.h"
 
 #include <type_traits>
 
 namespace {
+
+using kgops::LaplacianOp;
+using kgops::Wave13ptOp;
 
 constexpr int TX = 64;   // one wave64 per x-row segment
 constexpr int TY = 4;    // 4 rows -> 256 threads per workgroup
@@ -90,27 +93,6 @@ void uxx1_k(int nx, int ny, int ns, int ntx, int nty, double c1, double c2, doub
 // ------------------------------------------------- plane-engine stencil ops
 // Each op computes the VEC points of one lane at plane k (see kgp::View).
 
-// laplacian/laplacian.c:93-97
-template <class C>
-struct LaplacianOp {
-    const double* src;
-    const double* side[1];
-    double* dst;
-    double alpha, beta;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
-                                        const bool (&xm)[C::VEC]) const
-    {
-        double r[C::VEC];
-#pragma unroll
-        for (int v = 0; v < C::VEC; v++)
-            r[v] = alpha * p.col(v, 0) + beta * (
-                p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
-                p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
-                p.col(v, 1) + p.col(v, -1));
-        kgp::store<C::VEC>(dst, ob, x, r, xm);
-    }
-};
-
 // gradient/gradient.c:95-97
 template <class C>
 struct GradientOp {
@@ -131,32 +113,6 @@ struct GradientOp {
         kgp::store<C::VEC>(ux, ob, x, rx, xm);
         kgp::store<C::VEC>(uy, ob, x, ry, xm);
         kgp::store<C::VEC>(uz, ob, x, rz, xm);
-    }
-};
-
-// wave13pt/wave13pt.c:640-651; main field w1, side stream w0
-template <class C>
-struct Wave13ptOp {
-    const double* src;
-    const double* side[1];
-    double* dst;
-    double m0, m1, m2;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
-                                        const bool (&xm)[C::VEC]) const
-    {
-        double r[C::VEC];
-#pragma unroll
-        for (int v = 0; v < C::VEC; v++)
-            r[v] = m0 * p.col(v, 0) - p.side(0, v) +
-                m1 * (
-                    p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
-                    p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
-                    p.col(v, 1) + p.col(v, -1)) +
-                m2 * (
-                    p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
-                    p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
-                    p.col(v, 2) + p.col(v, -2));
-        kgp::store<C::VEC>(dst, ob, x, r, xm);
     }
 };
 

@@ -1,0 +1,299 @@
+// kg_typed.hip -- precision variants beside the FP64 drop-in kernels, and the
+// FP64 field statistics every precision run reports.
+//
+// laplacian and wave13pt with FP32 storage and FP32 arithmetic, or BF16
+// storage (bfloat16 bit patterns) and FP32 arithmetic; gaussblur applied per
+// xy-plane of a 3D grid in FP64 and FP32.  Same plane engine, same stencil
+// bodies (kg_ops3d.h) and same source order as the FP64 kernels; only the
+// types change.  Bytes per point drop from 16/24 (FP64) to 8/12 (FP32) and
+// 4/6 (BF16).
+//
+// kg_field_stats accumulates sums, sums of squares and maxima in FP64 over a
+// fixed partition (KG_STATS_BLOCKS workgroups, then one workgroup over their
+// partials), so a result does not depend on timing and repeats bit for bit.
+#include "kg_ops3d.h"
+
+#include <type_traits>
+
+namespace {
+
+using namespace kgops;
+
+#define KG_REQUIRE(cond, msg) \
+    do { if (!(cond)) return kg_set_error(KG_ERR_INVALID, "%s", msg); } while (0)
+
+// Single-grid launch of op Op in storage type T: 16-byte lanes (VEC = VMAX)
+// when nx is a multiple of VMAX, else one point per lane.
+template <template <class> class OpT, class T, int VMAX, int W, int D, int KC, class Fill>
+static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, Fill fill)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+    constexpr int LO = Bounds<OpT>::LO, HI = Bounds<OpT>::HI;
+    constexpr int ZLO = Bounds<OpT>::ZLO, ZHI = Bounds<OpT>::ZHI;
+    auto go = [&](auto vtag) -> int {
+        constexpr int V = decltype(vtag)::value;
+        using C = OpCfg<OpT, T, V, W, D, KC>;
+        using Op = OpT<C>;
+        Op op{};
+        fill(op);
+        const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI, ZLO, ZHI, KC);
+        auto kern = kgp::plane_kernel<C, LO, HI, Op, ZLO, ZHI>;
+        kg_register_kernel(name, (const void*)kern);
+        if (int rc = kg_launch_begin(ctx, name)) return rc;
+        if (!g.empty)
+            hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), 0, ctx->stream,
+                               nx, ny, ns, g.ntx, g.nty, op);
+        return kg_launch_end(ctx);
+    };
+    if (nx % VMAX == 0) return go(std::integral_constant<int, VMAX>{});
+    return go(std::integral_constant<int, 1>{});
+}
+
+// ------------------------------------------------------------ statistics
+constexpr int KG_STATS_BLOCKS = 1024;
+constexpr int KG_STATS_THREADS = 256;
+
+template <int DT>
+__device__ __forceinline__ double ld_any(const void* p, size_t i)
+{
+    if constexpr (DT == KG_F64) return ((const double*)p)[i];
+    else if constexpr (DT == KG_F32) return (double)((const float*)p)[i];
+    else return (double)kgp::bf16_in(((const uint16_t*)p)[i]);
+}
+
+// Per-block partials: [0] sum x, [1] sum x^2, [2] max |x|, [3] sum (x-r)^2,
+// [4] max |x-r|, written to part[block*5 + q].
+template <int DX, int DR>
+__global__ __launch_bounds__(KG_STATS_THREADS)
+void stats_k(size_t n, const void* __restrict__ x, const void* __restrict__ r,
+             double* __restrict__ part)
+{
+    __shared__ double red[5][KG_STATS_THREADS];
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+    const size_t stride = (size_t)gridDim.x * KG_STATS_THREADS;
+    for (size_t i = (size_t)blockIdx.x * KG_STATS_THREADS + threadIdx.x; i < n; i += stride) {
+        const double v = ld_any<DX>(x, i);
+        a0 += v;
+        a1 += v * v;
+        a2 = fmax(a2, fabs(v));
+        if (DR >= 0) {
+            const double d = v - ld_any<(DR < 0 ? 0 : DR)>(r, i);
+            a3 += d * d;
+            a4 = fmax(a4, fabs(d));
+        }
+    }
+    const int t = threadIdx.x;
+    red[0][t] = a0;
+    red[1][t] = a1;
+    red[2][t] = a2;
+    red[3][t] = a3;
+    red[4][t] = a4;
+    __syncthreads();
+    for (int h = KG_STATS_THREADS / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            red[0][t] += red[0][t + h];
+            red[1][t] += red[1][t + h];
+            red[2][t] = fmax(red[2][t], red[2][t + h]);
+            red[3][t] += red[3][t + h];
+            red[4][t] = fmax(red[4][t], red[4][t + h]);
+        }
+        __syncthreads();
+    }
+    if (t < 5) part[blockIdx.x * 5 + t] = red[t][0];
+}
+
+__global__ __launch_bounds__(KG_STATS_THREADS)
+void stats_final_k(int nblocks, const double* __restrict__ part, double* __restrict__ out)
+{
+    __shared__ double red[5][KG_STATS_THREADS];
+    const int t = threadIdx.x;
+    double a[5] = {0, 0, 0, 0, 0};
+    for (int b = t; b < nblocks; b += KG_STATS_THREADS) {
+        a[0] += part[b * 5 + 0];
+        a[1] += part[b * 5 + 1];
+        a[2] = fmax(a[2], part[b * 5 + 2]);
+        a[3] += part[b * 5 + 3];
+        a[4] = fmax(a[4], part[b * 5 + 4]);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) red[q][t] = a[q];
+    __syncthreads();
+    for (int h = KG_STATS_THREADS / 2; h > 0; h >>= 1) {
+        if (t < h) {
+            red[0][t] += red[0][t + h];
+            red[1][t] += red[1][t + h];
+            red[2][t] = fmax(red[2][t], red[2][t + h]);
+            red[3][t] += red[3][t + h];
+            red[4][t] = fmax(red[4][t], red[4][t + h]);
+        }
+        __syncthreads();
+    }
+    if (t < 5) out[t] = red[t][0];
+}
+
+template <int DX, int DR>
+static void launch_stats(hipStream_t st, size_t n, const void* x, const void* r, double* part,
+                         double* out)
+{
+    hipLaunchKernelGGL((stats_k<DX, DR>), dim3(KG_STATS_BLOCKS), dim3(KG_STATS_THREADS), 0, st, n,
+                       x, r, part);
+    hipLaunchKernelGGL(stats_final_k, dim3(1), dim3(KG_STATS_THREADS), 0, st, KG_STATS_BLOCKS,
+                       part, out);
+}
+
+template <int DX>
+static int stats_dispatch_ref(hipStream_t st, int rdt, size_t n, const void* x, const void* r,
+                              double* part, double* out)
+{
+    if (!r) launch_stats<DX, -1>(st, n, x, r, part, out);
+    else if (rdt == KG_F64) launch_stats<DX, KG_F64>(st, n, x, r, part, out);
+    else if (rdt == KG_F32) launch_stats<DX, KG_F32>(st, n, x, r, part, out);
+    else if (rdt == KG_BF16) launch_stats<DX, KG_BF16>(st, n, x, r, part, out);
+    else return kg_set_error(KG_ERR_INVALID, "kg_field_stats: unknown reference dtype %d", rdt);
+    return KG_OK;
+}
+
+}  // namespace
+
+// ======================================================= precision variants
+
+extern "C" int kg_laplacian_f32(kg_ctx* ctx, int nx, int ny, int ns, float alpha, float beta,
+                                const float* w0, float* w1)
+{
+    KG_REQUIRE(w0 && w1, "kg_laplacian_f32: null buffer");
+    KG_REQUIRE((const void*)w0 != (const void*)w1, "kg_laplacian_f32: w0 and w1 must be distinct");
+    return launch_typed<LaplacianOp, kgp::F32, 4, 8, 2, 16>(ctx, "laplacian_f32", nx, ny, ns,
+                                                            [&](auto& op) {
+        op.src = w0;
+        op.dst = w1;
+        op.alpha = alpha;
+        op.beta = beta;
+    });
+}
+
+extern "C" int kg_laplacian_bf16(kg_ctx* ctx, int nx, int ny, int ns, float alpha, float beta,
+                                 const uint16_t* w0, uint16_t* w1)
+{
+    KG_REQUIRE(w0 && w1, "kg_laplacian_bf16: null buffer");
+    KG_REQUIRE((const void*)w0 != (const void*)w1, "kg_laplacian_bf16: w0 and w1 must be distinct");
+    return launch_typed<LaplacianOp, kgp::BF16, 8, 8, 2, 16>(ctx, "laplacian_bf16", nx, ny, ns,
+                                                             [&](auto& op) {
+        op.src = w0;
+        op.dst = w1;
+        op.alpha = alpha;
+        op.beta = beta;
+    });
+}
+
+extern "C" int kg_wave13pt_f32(kg_ctx* ctx, int nx, int ny, int ns, float m0, float m1, float m2,
+                               const float* w0, const float* w1, float* w2)
+{
+    KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt_f32: null buffer");
+    KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_f32: w2 aliases an input");
+    return launch_typed<Wave13ptOp, kgp::F32, 4, 16, 2, 32>(ctx, "wave13pt_f32", nx, ny, ns,
+                                                            [&](auto& op) {
+        op.src = w1;
+        op.side[0] = w0;
+        op.dst = w2;
+        op.m0 = m0;
+        op.m1 = m1;
+        op.m2 = m2;
+    });
+}
+
+extern "C" int kg_wave13pt_bf16(kg_ctx* ctx, int nx, int ny, int ns, float m0, float m1, float m2,
+                                const uint16_t* w0, const uint16_t* w1, uint16_t* w2)
+{
+    KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt_bf16: null buffer");
+    KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_bf16: w2 aliases an input");
+    // 8-byte lanes: with 16-byte lanes (8 points) the 1024-thread tile needs
+    // more than the 128 registers a lane may hold at that size and spills.
+    return launch_typed<Wave13ptOp, kgp::BF16, 4, 16, 2, 32>(ctx, "wave13pt_bf16", nx, ny, ns,
+                                                             [&](auto& op) {
+        op.src = w1;
+        op.side[0] = w0;
+        op.dst = w2;
+        op.m0 = m0;
+        op.m1 = m1;
+        op.m2 = m2;
+    });
+}
+
+extern "C" int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0, double s1,
+                                 double s2, double s4, double s5, double s8, const double* w0,
+                                 double* w1)
+{
+    KG_REQUIRE(w0 && w1, "kg_gaussblur3_f64: null buffer");
+    KG_REQUIRE(w0 != w1, "kg_gaussblur3_f64: w0 and w1 must be distinct");
+    const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
+    return launch_typed<Gaussblur3Op, kgp::F64, 2, 8, 2, 16>(ctx, "gaussblur3_f64", nx, ny, ns,
+                                                             [&](auto& op) {
+        op.src = w0;
+        op.dst = w1;
+        op.f = f;
+        op.s0 = s0;
+        op.s1 = s1;
+        op.s2 = s2;
+        op.s4 = s4;
+        op.s5 = s5;
+        op.s8 = s8;
+    });
+}
+
+extern "C" int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, float s1,
+                                 float s2, float s4, float s5, float s8, const float* w0, float* w1)
+{
+    KG_REQUIRE(w0 && w1, "kg_gaussblur3_f32: null buffer");
+    KG_REQUIRE(w0 != w1, "kg_gaussblur3_f32: w0 and w1 must be distinct");
+    const float f = 1.f / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);
+    return launch_typed<Gaussblur3Op, kgp::F32, 4, 8, 2, 16>(ctx, "gaussblur3_f32", nx, ny, ns,
+                                                             [&](auto& op) {
+        op.src = w0;
+        op.dst = w1;
+        op.f = f;
+        op.s0 = s0;
+        op.s1 = s1;
+        op.s2 = s2;
+        op.s4 = s4;
+        op.s5 = s5;
+        op.s8 = s8;
+    });
+}
+
+// ================================================================ statistics
+
+extern "C" int kg_field_stats(kg_ctx* ctx, int dtype, size_t n, const void* x, int ref_dtype,
+                              const void* ref, double* out)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    KG_REQUIRE(out, "kg_field_stats: null output");
+    KG_REQUIRE(n == 0 || x, "kg_field_stats: null field");
+    KG_REQUIRE(dtype == KG_F64 || dtype == KG_F32 || dtype == KG_BF16,
+               "kg_field_stats: unknown dtype");
+    KG_REQUIRE(!ref || ref_dtype == KG_F64 || ref_dtype == KG_F32 || ref_dtype == KG_BF16,
+               "kg_field_stats: unknown reference dtype");
+    int rc = KG_OK;
+    double* dev = ctx->stats;   // KG_STATS_BLOCKS*5 partials followed by 5 results
+    double* res = dev + KG_STATS_BLOCKS * 5;
+    kg_register_kernel("field_stats", (const void*)stats_final_k);
+    if ((rc = kg_launch_begin(ctx, "field_stats"))) return rc;
+    switch (dtype) {
+    case KG_F64: rc = stats_dispatch_ref<KG_F64>(ctx->stream, ref_dtype, n, x, ref, dev, res); break;
+    case KG_F32: rc = stats_dispatch_ref<KG_F32>(ctx->stream, ref_dtype, n, x, ref, dev, res); break;
+    case KG_BF16: rc = stats_dispatch_ref<KG_BF16>(ctx->stream, ref_dtype, n, x, ref, dev, res); break;
+    default: rc = kg_set_error(KG_ERR_INVALID, "kg_field_stats: unknown dtype %d", dtype);
+    }
+    if (rc) return rc;
+    if ((rc = kg_launch_end(ctx))) return rc;
+    KG_HIP(hipMemcpyAsync(out, res, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP(hipStreamSynchronize(ctx->stream));
+    return KG_OK;
+}
+
+size_t kg_stats_scratch_bytes() { return (KG_STATS_BLOCKS * 5 + 5) * sizeof(double); }
+
+int kg_preload_typed()
+{
+    return kg_preload_kernel((const void*)stats_final_k);
+}

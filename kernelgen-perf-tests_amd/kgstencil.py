@@ -19,6 +19,10 @@ LIB_PATH = PKG_DIR / "lib" / "libkgstencil.so"
 _dp = C.c_void_p
 _i = C.c_int
 _d = C.c_double
+_f = C.c_float
+
+# kg_dtype codes
+F64, F32, BF16 = 0, 1, 2
 
 # name -> argtypes after the context pointer (None marks a device pointer)
 SIGNATURES = {
@@ -34,6 +38,14 @@ SIGNATURES = {
     "kg_gaussblur": [_i, _i, _d, _d, _d, _d, _d, _d, _dp, _dp],
     "kg_whispering": [_i, _i, _d, _d] + [_dp] * 10,
     "kg_gameoflife": [_i, _i, _dp, _dp],
+    "kg_life_u8": [_i, _i, _dp, _dp],
+    "kg_laplacian_f32": [_i, _i, _i, _f, _f, _dp, _dp],
+    "kg_laplacian_bf16": [_i, _i, _i, _f, _f, _dp, _dp],
+    "kg_wave13pt_f32": [_i, _i, _i, _f, _f, _f, _dp, _dp, _dp],
+    "kg_wave13pt_bf16": [_i, _i, _i, _f, _f, _f, _dp, _dp, _dp],
+    "kg_gaussblur3_f64": [_i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _dp],
+    "kg_gaussblur3_f32": [_i, _i, _i, _f, _f, _f, _f, _f, _f, _dp, _dp],
+    "kg_field_stats": [_i, C.c_size_t, _dp, _i, _dp, C.POINTER(_d)],
     "kg_matmul": [_i, _i, _i, _dp, _dp, _dp],
     "kg_stream_copy": [_dp, _dp, C.c_size_t],
 }
@@ -200,6 +212,19 @@ class Context:
         _check(self.lib, self.lib.kg_kernel_regcount(self.ptr, kernel.encode(), C.byref(n)),
                "kg_kernel_regcount")
         return n.value
+
+    def field_stats(self, x: "DeviceArray", dtype: int, ref: "DeviceArray | None" = None,
+                    ref_dtype: int = F64, n: int | None = None) -> dict:
+        """FP64 sum, sum of squares, max |x|, and (with ref) sum (x-ref)^2 and
+        max |x-ref| of a device field (kg_field_stats)."""
+        out = (_d * 5)()
+        if n is None:
+            n = int(np.prod(x.shape))
+        rc = self.lib.kg_field_stats(self.ptr, dtype, n, x.ptr, ref_dtype,
+                                     ref.ptr if ref is not None else None, out)
+        _check(self.lib, rc, "kg_field_stats")
+        return {"sum": out[0], "sumsq": out[1], "max": out[2], "err_sumsq": out[3],
+                "err_max": out[4]}
 
     def call(self, fn: str, *args) -> None:
         """Call kg_<fn>; DeviceArray arguments are passed as device pointers."""

@@ -363,3 +363,103 @@ double oracle_sum(size_t n, const double* a)
         s += a[i];
     return s;
 }
+
+/* ---- precision variants: the same stencils in FP32 arithmetic ------------
+ * FP32 storage (_f32) or bfloat16 storage (_bf16: uint16_t bit patterns,
+ * widened exactly to float at load and rounded to nearest even at store).
+ * Operand order as in the FP64 functions above (laplacian/laplacian.c:93-97,
+ * wave13pt/wave13pt.c:640-651, gaussblur/gaussblur.c:65,85-92). */
+
+float oracle_bf16_to_f32(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, sizeof f);
+    return f;
+}
+
+uint16_t oracle_f32_to_bf16(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u); /* quiet NaN */
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);                 /* nearest even */
+}
+
+#define LD_F32(a, c) ((a)[c])
+#define ST_F32(a, c, v) ((a)[c] = (v))
+#define LD_BF16(a, c) oracle_bf16_to_f32((a)[c])
+#define ST_BF16(a, c, v) ((a)[c] = oracle_f32_to_bf16(v))
+
+#define DEF_LAPLACIAN(SUFFIX, ST, LD, STORE)                                              \
+    void oracle_laplacian_##SUFFIX(int nx, int ny, int ns, float alpha, float beta,       \
+                                   const ST* w0, ST* w1)                                  \
+    {                                                                                     \
+        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;                               \
+        for (int k = 1; k < ns - 1; k++)                                                  \
+            for (int j = 1; j < ny - 1; j++)                                              \
+                for (int i = 1; i < nx - 1; i++) {                                        \
+                    size_t c = IDX3(i, j, k);                                             \
+                    float r = alpha * LD(w0, c) + beta * (                                \
+                        LD(w0, c + 1) + LD(w0, c - 1) +                                   \
+                        LD(w0, c + sy) + LD(w0, c - sy) +                                 \
+                        LD(w0, c + sz) + LD(w0, c - sz));                                 \
+                    STORE(w1, c, r);                                                      \
+                }                                                                         \
+    }
+DEF_LAPLACIAN(f32, float, LD_F32, ST_F32)
+DEF_LAPLACIAN(bf16, uint16_t, LD_BF16, ST_BF16)
+
+#define DEF_WAVE(SUFFIX, ST, LD, STORE)                                                   \
+    void oracle_wave13pt_##SUFFIX(int nx, int ny, int ns, float m0, float m1, float m2,   \
+                                  const ST* w0, const ST* w1, ST* w2)                     \
+    {                                                                                     \
+        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;                               \
+        for (int k = 2; k < ns - 2; k++)                                                  \
+            for (int j = 2; j < ny - 2; j++)                                              \
+                for (int i = 2; i < nx - 2; i++) {                                        \
+                    size_t c = IDX3(i, j, k);                                             \
+                    float r = m0 * LD(w1, c) - LD(w0, c) +                                \
+                        m1 * (                                                            \
+                            LD(w1, c + 1) + LD(w1, c - 1) +                               \
+                            LD(w1, c + sy) + LD(w1, c - sy) +                             \
+                            LD(w1, c + sz) + LD(w1, c - sz)) +                            \
+                        m2 * (                                                            \
+                            LD(w1, c + 2) + LD(w1, c - 2) +                               \
+                            LD(w1, c + 2 * sy) + LD(w1, c - 2 * sy) +                     \
+                            LD(w1, c + 2 * sz) + LD(w1, c - 2 * sz));                     \
+                    STORE(w2, c, r);                                                      \
+                }                                                                         \
+    }
+DEF_WAVE(f32, float, LD_F32, ST_F32)
+DEF_WAVE(bf16, uint16_t, LD_BF16, ST_BF16)
+
+/* gaussblur/gaussblur.c:85-92 on every xy-plane k in [0,ns): xy-interior
+ * [2,n-2). */
+#define DEF_GAUSS3(SUFFIX, RT)                                                            \
+    void oracle_gaussblur3_##SUFFIX(int nx, int ny, int ns, RT s0, RT s1, RT s2, RT s4,   \
+                                    RT s5, RT s8, const RT* w0, RT* w1)                   \
+    {                                                                                     \
+        const RT f = (RT)1 / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);                     \
+        for (int k = 0; k < ns; k++)                                                      \
+            for (int j = 2; j < ny - 2; j++)                                              \
+                for (int i = 2; i < nx - 2; i++) {                                        \
+                    const RT* p = w0 + IDX3(0, 0, k);                                     \
+                    w1[IDX3(i, j, k)] = f * (                                             \
+                        s0 * p[IDX2(i, j)] +                                              \
+                        s1 * (p[IDX2(i - 1, j)] + p[IDX2(i + 1, j)] +                     \
+                              p[IDX2(i, j - 1)] + p[IDX2(i, j + 1)]) +                    \
+                        s2 * (p[IDX2(i - 1, j - 1)] + p[IDX2(i + 1, j - 1)] +             \
+                              p[IDX2(i - 1, j + 1)] + p[IDX2(i + 1, j + 1)]) +             \
+                        s4 * (p[IDX2(i - 2, j)] + p[IDX2(i + 2, j)] +                     \
+                              p[IDX2(i, j - 2)] + p[IDX2(i, j + 2)]) +                    \
+                        s5 * (p[IDX2(i - 2, j - 1)] + p[IDX2(i - 1, j - 2)] +             \
+                              p[IDX2(i + 1, j - 2)] + p[IDX2(i + 2, j - 1)] +             \
+                              p[IDX2(i - 2, j + 1)] + p[IDX2(i - 1, j + 2)] +             \
+                              p[IDX2(i + 1, j + 2)] + p[IDX2(i + 2, j + 1)]) +             \
+                        s8 * (p[IDX2(i - 2, j - 2)] + p[IDX2(i + 2, j - 2)] +             \
+                              p[IDX2(i - 2, j + 2)] + p[IDX2(i + 2, j + 2)]));            \
+                }                                                                         \
+    }
+DEF_GAUSS3(f64, double)
+DEF_GAUSS3(f32, float)

@@ -93,6 +93,24 @@ void oracle_matmul(int nx, int ny, int ns, const double* A, const double* B, dou
  * gameoflife): cells 0/1, fixed dead border outside the board. */
 void oracle_life_u8(int nx, int ny, const uint8_t* c0, uint8_t* c1);
 
+/* Precision variants (FP32 arithmetic; _bf16 = bfloat16 storage as uint16_t
+ * bit patterns, rounded to nearest even) and gaussblur per xy-plane of a 3D
+ * grid (all planes, xy-interior [2,n-2)). */
+float oracle_bf16_to_f32(uint16_t h);
+uint16_t oracle_f32_to_bf16(float f);
+void oracle_laplacian_f32(int nx, int ny, int ns, float alpha, float beta,
+                          const float* w0, float* w1);
+void oracle_laplacian_bf16(int nx, int ny, int ns, float alpha, float beta,
+                           const uint16_t* w0, uint16_t* w1);
+void oracle_wave13pt_f32(int nx, int ny, int ns, float m0, float m1, float m2,
+                         const float* w0, const float* w1, float* w2);
+void oracle_wave13pt_bf16(int nx, int ny, int ns, float m0, float m1, float m2,
+                          const uint16_t* w0, const uint16_t* w1, uint16_t* w2);
+void oracle_gaussblur3_f64(int nx, int ny, int ns, double s0, double s1, double s2,
+                           double s4, double s5, double s8, const double* w0, double* w1);
+void oracle_gaussblur3_f32(int nx, int ny, int ns, float s0, float s1, float s2,
+                           float s4, float s5, float s8, const float* w0, float* w1);
+
 /* glibc rand() based fill in the reference's order: values are drawn
  * interleaved per index, arrays[0][i], arrays[1][i], ..., (e.g.
  * laplacian/laplacian.c:158-164).  real_rand() = ((rand()/RAND_MAX)-0.5)*2

@@ -245,6 +245,36 @@ def test_life_u8_is_conway(oracle):
     np.testing.assert_array_equal(out, _conway(b))
 
 
+def test_life_u8_known_patterns(oracle):
+    """Textbook answers: a blinker has period 2 (vertical <-> horizontal), a
+    block is still, a glider moves (+1, +1) every 4 generations."""
+    lib = oracle.lib
+
+    def steps(board, nt):
+        ny, nx = board.shape
+        cur = board.copy()
+        for _ in range(nt):
+            nxt = cur.copy()
+            lib.oracle_life_u8(C.c_int(nx), C.c_int(ny), C.c_void_p(cur.ctypes.data),
+                               C.c_void_p(nxt.ctypes.data))
+            cur = nxt
+        return cur
+
+    b = np.zeros((9, 9), np.uint8)
+    b[3:6, 4] = 1
+    h = np.zeros_like(b)
+    h[4, 3:6] = 1
+    np.testing.assert_array_equal(steps(b, 1), h)
+    np.testing.assert_array_equal(steps(b, 2), b)
+    blk = np.zeros((6, 6), np.uint8)
+    blk[2:4, 2:4] = 1
+    np.testing.assert_array_equal(steps(blk, 3), blk)
+    g = np.zeros((20, 20), np.uint8)
+    for (y, x) in [(1, 2), (2, 3), (3, 1), (3, 2), (3, 3)]:
+        g[2 + y, 2 + x] = 1
+    np.testing.assert_array_equal(steps(g, 4), np.roll(g, (1, 1), axis=(0, 1)))
+
+
 def test_whispering_closed_form_equals_serial_sweep(oracle):
     """The GPU formulation (no in-place dependencies) reproduces the serial
     in-place sweep of whispering/whispering.c:87-106 bit for bit."""
