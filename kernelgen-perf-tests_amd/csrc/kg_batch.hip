@@ -133,7 +133,7 @@ struct BatchState {
 __shared__ __attribute__((aligned(16))) char batch_lds[BATCH_LDS];
 
 template <int K, int DT, int LPR>
-__device__ __forceinline__ void run_tile(const GridDesc* __restrict__ gp, int bx, int by,
+__device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ gp, int bx, int by,
                                                    int bz, unsigned long long step)
 {
     const GridDesc& g = *gp;

@@ -72,6 +72,17 @@ __device__ __forceinline__ typename T::S st_cvt(typename T::F f)
     else return f;
 }
 
+// Global-memory (address space 1) view of a pointer.  Pointers that a kernel
+// loads from memory (the batched kernel's grid descriptors) are generic to the
+// compiler, which would emit flat loads and stores through them; accessing
+// through an address-space-1 pointer gives global instructions.
+template <class T> using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem<T>* as_global(T* p)
+{
+    return (gmem<T>*)p;
+}
+
 // An unsigned vector of BYTES bytes (one global access).
 template <int BYTES> struct Raw;
 template <> struct Raw<16> { typedef unsigned t __attribute__((ext_vector_type(4))); };
@@ -81,11 +92,11 @@ template <> struct Raw<2> { typedef unsigned short t; };
 
 // VEC consecutive storage values at p (aligned to the access size).
 template <class T, int VEC>
-__device__ __forceinline__ void vload(const typename T::S* p, typename T::F (&out)[VEC])
+__device__ __forceinline__ void vload(const gmem<typename T::S>* p, typename T::F (&out)[VEC])
 {
     using S = typename T::S;
     using R = typename Raw<VEC * (int)sizeof(S)>::t;
-    const R r = *(const R*)p;
+    const R r = *(const gmem<R>*)p;
     S s[VEC];
     __builtin_memcpy(s, &r, sizeof r);
 #pragma unroll
@@ -220,11 +231,11 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) p_ok[v] = row_ok && x + v < nx;
     const bool vec_ok = C::VEC > 1 && p_ok[C::VEC - 1];
-    const S* src = op.src + (long long)j * sy;
+    const gmem<const S>* src = as_global(op.src) + (long long)j * sy;
 
     auto load = [&](int k, Slot<F, C::VEC>& s) {
         if (k < 0 || k >= ns) return;
-        const S* pp = src + (long long)k * sz;
+        const gmem<const S>* pp = src + (long long)k * sz;
         if (C::VEC > 1 && vec_ok) {
             vload<T, C::VEC>(pp + x, s.v);
         } else {
@@ -237,7 +248,7 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
         if (C::NAUX == 0 || !comp_row || k >= ns) return;
 #pragma unroll
         for (int a = 0; a < C::NAUX; a++) {
-            const S* pp = op.side[a] + (long long)j * sy + (long long)k * sz;
+            const gmem<const S>* pp = as_global(op.side[a]) + (long long)j * sy + (long long)k * sz;
             if (C::VEC > 1 && vec_ok) {
                 vload<T, C::VEC>(pp + x, s.a[a]);
             } else {
@@ -332,6 +343,7 @@ __device__ __forceinline__ void store(typename T::S* dst, long long ob, int x,
                                       const typename T::F (&r)[VEC], const bool (&xmask)[VEC])
 {
     using S = typename T::S;
+    gmem<S>* d = as_global(dst);
     bool all = true;
 #pragma unroll
     for (int v = 0; v < VEC; v++) all = all && xmask[v];
@@ -342,11 +354,11 @@ __device__ __forceinline__ void store(typename T::S* dst, long long ob, int x,
         for (int v = 0; v < VEC; v++) s[v] = st_cvt<T>(r[v]);
         R raw;
         __builtin_memcpy(&raw, s, sizeof raw);
-        kg_st((R*)(dst + ob + x), raw);
+        __builtin_nontemporal_store(raw, (gmem<R>*)(d + ob + x));
     } else {
 #pragma unroll
         for (int v = 0; v < VEC; v++)
-            if (xmask[v]) kg_st(dst + ob + x + v, st_cvt<T>(r[v]));
+            if (xmask[v]) __builtin_nontemporal_store(st_cvt<T>(r[v]), d + ob + x + v);
     }
 }
 template <int VEC>

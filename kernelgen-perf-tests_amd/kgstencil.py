@@ -93,6 +93,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.kg_last_kernel_time.argtypes = [_dp, C.POINTER(_d)]
     lib.kg_timing_drain.argtypes = [_dp, C.POINTER(_d), _i, C.POINTER(_i)]
     lib.kg_kernel_regcount.argtypes = [_dp, C.c_char_p, C.POINTER(_i)]
+    _declare_batch(lib)
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.argtypes = [_dp] + args
@@ -231,3 +232,148 @@ class Context:
         conv = [a.ptr if isinstance(a, DeviceArray) else a for a in args]
         rc = getattr(self.lib, "kg_" + fn)(self.ptr, *conv)
         _check(self.lib, rc, "kg_" + fn)
+
+
+# --------------------------------------------------------------- batches
+KIND_LAPLACIAN, KIND_WAVE13PT, KIND_GAUSSBLUR3 = 0, 1, 2
+KINDS = {"laplacian": KIND_LAPLACIAN, "wave13pt": KIND_WAVE13PT, "gaussblur3": KIND_GAUSSBLUR3}
+NLEV = {KIND_LAPLACIAN: 2, KIND_WAVE13PT: 3, KIND_GAUSSBLUR3: 2}
+
+
+class GridSpec(C.Structure):
+    _fields_ = [("kind", _i), ("dtype", _i), ("nx", _i), ("ny", _i), ("ns", _i),
+                ("coef", _d * 6)]
+
+
+_BATCH_SIGS = {
+    "kg_batch_create": [_dp, C.POINTER(_dp)],
+    "kg_batch_destroy": [_dp],
+    "kg_batch_add": [_dp, C.POINTER(GridSpec), C.POINTER(_i)],
+    "kg_batch_remove": [_dp, _i],
+    "kg_batch_upload": [_dp, _i, _i, _dp],
+    "kg_batch_download": [_dp, _i, _i, _dp],
+    "kg_batch_newest_level": [_dp, _i, C.POINTER(_i)],
+    "kg_batch_steps_done": [_dp, _i, C.POINTER(C.c_longlong)],
+    "kg_batch_step": [_dp, _i, _i],
+    "kg_batch_info": [_dp, C.POINTER(_d), C.POINTER(_d), C.POINTER(_i), C.POINTER(_i)],
+    "kg_batch_tune": [_dp, _i, C.POINTER(_i)],
+    "kg_batch_grid_shape": [_dp, _i, C.POINTER(_i), C.POINTER(_i)],
+    "kg_batch_set_shape": [_dp, _i, _i, _i],
+    "kg_batch_memory": [_dp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+}
+EXPORTS += list(_BATCH_SIGS)
+
+
+def _declare_batch(lib):
+    if getattr(lib, "_kg_batch_declared", False):
+        return
+    for name, args in _BATCH_SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _i
+    lib._kg_batch_declared = True
+
+
+class Batch:
+    """Many independent 3D grids stepped together (kg_batch_* of the C ABI).
+
+    Host arrays are dense (ns, ny, nx) in the grid's dtype (float64/float32).
+    """
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        _declare_batch(self.lib)
+        p = _dp()
+        _check(self.lib, self.lib.kg_batch_create(ctx.ptr, C.byref(p)), "kg_batch_create")
+        self.ptr = p.value
+        self.specs: dict[int, tuple] = {}
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.kg_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, kind: str, dtype: int, dims, coefs) -> int:
+        s = GridSpec()
+        s.kind, s.dtype = KINDS[kind], dtype
+        s.nx, s.ny, s.ns = (int(d) for d in dims)
+        for q, c in enumerate(coefs):
+            s.coef[q] = float(c)
+        gid = _i(0)
+        _check(self.lib, self.lib.kg_batch_add(self.ptr, C.byref(s), C.byref(gid)), "kg_batch_add")
+        self.specs[gid.value] = (kind, dtype, tuple(int(d) for d in dims))
+        return gid.value
+
+    def remove(self, gid: int) -> None:
+        _check(self.lib, self.lib.kg_batch_remove(self.ptr, gid), "kg_batch_remove")
+        del self.specs[gid]
+
+    def _np_dtype(self, gid):
+        return np.float64 if self.specs[gid][1] == F64 else np.float32
+
+    def upload(self, gid: int, level: int, a: np.ndarray) -> None:
+        kind, dt, (nx, ny, ns) = self.specs[gid]
+        a = np.ascontiguousarray(a, dtype=self._np_dtype(gid))
+        if a.size != nx * ny * ns:
+            raise ValueError("upload: wrong size")
+        _check(self.lib, self.lib.kg_batch_upload(self.ptr, gid, level, a.ctypes.data),
+               "kg_batch_upload")
+
+    def download(self, gid: int, level: int | None = None) -> np.ndarray:
+        kind, dt, (nx, ny, ns) = self.specs[gid]
+        if level is None:
+            level = self.newest_level(gid)
+        out = np.empty((ns, ny, nx), dtype=self._np_dtype(gid))
+        _check(self.lib, self.lib.kg_batch_download(self.ptr, gid, level, out.ctypes.data),
+               "kg_batch_download")
+        return out
+
+    def newest_level(self, gid: int) -> int:
+        v = _i(0)
+        _check(self.lib, self.lib.kg_batch_newest_level(self.ptr, gid, C.byref(v)),
+               "kg_batch_newest_level")
+        return v.value
+
+    def steps_done(self, gid: int) -> int:
+        v = C.c_longlong(0)
+        _check(self.lib, self.lib.kg_batch_steps_done(self.ptr, gid, C.byref(v)),
+               "kg_batch_steps_done")
+        return v.value
+
+    def step(self, n: int = 1, graph: bool = True) -> None:
+        _check(self.lib, self.lib.kg_batch_step(self.ptr, n, int(graph)), "kg_batch_step")
+
+    def info(self) -> dict:
+        p, b, it, wg = _d(), _d(), _i(), _i()
+        _check(self.lib, self.lib.kg_batch_info(self.ptr, C.byref(p), C.byref(b), C.byref(it),
+                                                C.byref(wg)), "kg_batch_info")
+        return {"points_per_step": p.value, "bytes_per_step": b.value, "work_items": it.value,
+                "workgroups": wg.value}
+
+    def tune(self, reps: int = 3) -> int:
+        n = _i(0)
+        _check(self.lib, self.lib.kg_batch_tune(self.ptr, reps, C.byref(n)), "kg_batch_tune")
+        return n.value
+
+    def shape(self, gid: int) -> tuple[int, int]:
+        a, b = _i(), _i()
+        _check(self.lib, self.lib.kg_batch_grid_shape(self.ptr, gid, C.byref(a), C.byref(b)),
+               "kg_batch_grid_shape")
+        return a.value, b.value
+
+    def set_shape(self, gid: int, lanes_per_row: int, planes_per_tile: int) -> None:
+        _check(self.lib, self.lib.kg_batch_set_shape(self.ptr, gid, lanes_per_row,
+                                                     planes_per_tile), "kg_batch_set_shape")
+
+    def memory(self) -> tuple[int, int]:
+        r, u = C.c_size_t(), C.c_size_t()
+        _check(self.lib, self.lib.kg_batch_memory(self.ptr, C.byref(r), C.byref(u)),
+               "kg_batch_memory")
+        return r.value, u.value
