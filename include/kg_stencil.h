@@ -175,6 +175,10 @@ int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0, double s1,
 int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, float s1, float s2,
                       float s4, float s5, float s8, const float* w0, float* w1);
 
+/* Fills n values (KG_F64 or KG_F32) at p with uniform [-1, 1) values from a
+ * counter-based hash of (seed, index); asynchronous on the context stream. */
+int kg_fill_uniform(kg_ctx* ctx, int dtype, size_t n, void* p, unsigned long long seed);
+
 /* FP64 statistics of a device field x of n values of type dtype, optionally
  * against a reference field ref of type ref_dtype (NULL: none).  out (host,
  * 5 doubles) = { sum x, sum x^2, max |x|, sum (x-ref)^2, max |x-ref| }.
@@ -203,14 +207,21 @@ int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, const double
  * written and keep what each level held.  kg_batch_newest_level tells which
  * level holds the latest values.  Host buffers are dense (x fastest);
  * device rows are padded to 16 bytes. */
-typedef enum { KG_KIND_LAPLACIAN = 0, KG_KIND_WAVE13PT = 1, KG_KIND_GAUSSBLUR3 = 2 } kg_kind;
+/* Kinds: the 3D stencils, and the 2D ones applied to every xy-plane (a 2D
+ * nx x ny problem is a grid with ns = 1): jacobi (jacobi/jacobi.F90:65-67,
+ * interior [1,n-1)) and gameoflife's FP rule (gameoflife/gameoflife.c:82-91). */
+typedef enum {
+    KG_KIND_LAPLACIAN = 0, KG_KIND_WAVE13PT = 1, KG_KIND_GAUSSBLUR3 = 2,
+    KG_KIND_JACOBI2 = 3, KG_KIND_LIFE2 = 4
+} kg_kind;
 
 typedef struct kg_grid_spec {
     int kind;        /* kg_kind */
     int dtype;       /* KG_F64 or KG_F32 */
     int nx, ny, ns;
     double coef[6];  /* laplacian: alpha, beta; wave13pt: m0, m1, m2;
-                        gaussblur3: s0, s1, s2, s4, s5, s8 */
+                        gaussblur3: s0, s1, s2, s4, s5, s8; jacobi2: c0, c1, c2;
+                        life2: none */
 } kg_grid_spec;
 
 typedef struct kg_batch kg_batch;
@@ -239,8 +250,56 @@ int kg_batch_tune(kg_batch* b, int reps, int* classes_tuned);
 int kg_batch_grid_shape(kg_batch* b, int id, int* lanes_per_row, int* planes_per_tile);
 /* Forces the tile shape of the size class of grid id. */
 int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int planes_per_tile);
+/* Fills every level of every grid on the device with uniform [-1, 1) values
+ * from a counter-based hash of (seed, grid, level, element). */
+int kg_batch_fill(kg_batch* b, unsigned long long seed);
+/* The comparison path: n steps as one plain launch per grid and step (same
+ * tile bodies and results as kg_batch_step). */
+int kg_batch_step_sequential(kg_batch* b, int nsteps);
 /* HBM reserved by the batch's arena and bytes held by live grids. */
 int kg_batch_memory(kg_batch* b, size_t* reserved, size_t* in_use);
+/* Fills the levels of one grid like kg_batch_fill (asynchronous); both levels
+ * of a jacobi2 grid get the same values (one Dirichlet boundary). */
+int kg_batch_fill_grid(kg_batch* b, int id, unsigned long long seed);
+
+/* ---- serving -------------------------------------------------------------------
+ * No reference counterpart.  A request loop over a batch: jobs arrive at
+ * given times, are admitted in arrival order while at most max_active run and
+ * their levels fit in hbm_budget bytes, stay resident until done, and advance
+ * one step per tick together (continuous batching: one persistent launch per
+ * tick over every active job).  KG_SERVE_SEQUENTIAL runs one job at a time
+ * with one plain launch per step, for comparison.  Initial data come from the
+ * job's seed (kg_batch_fill_grid).  A jacobi2 job with tol > 0 stops early
+ * once max |w_new - w_old| < tol (checked every 10 steps). */
+enum { KG_SERVE_CONTINUOUS = 0, KG_SERVE_SEQUENTIAL = 1 };
+
+typedef struct kg_job {
+    int kind, dtype, nx, ny, ns;
+    int steps;
+    double tol;
+    double coef[6];
+    double arrival;            /* seconds after the start of the run */
+    unsigned long long seed;
+    void* out;                 /* optional host buffer (dense nx*ny*ns) for the final field */
+} kg_job;
+
+typedef struct kg_job_result {
+    double admitted, finished; /* seconds after the start of the run */
+    long long steps_run;
+    double checksum[2];        /* FP64 sum and sum of squares of the final field's level */
+    int status;                /* KG_OK, or KG_ERR_NOMEM if it exceeds the budget */
+} kg_job_result;
+
+typedef struct kg_serve_stats {
+    double wall;               /* seconds */
+    double points;             /* grid points x steps computed */
+    double p50, p99, mean_latency;   /* finished - arrival, seconds */
+    int peak_active;
+    size_t peak_bytes;
+} kg_serve_stats;
+
+int kg_serve_run(kg_ctx* ctx, const kg_job* jobs, int njobs, size_t hbm_budget, int max_active,
+                 int mode, kg_job_result* results, kg_serve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -24,8 +24,11 @@
 #include "kg_ops3d.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <thread>
 #include <new>
 #include <vector>
 
@@ -36,7 +39,7 @@ using namespace kgops;
 // ------------------------------------------------------------ variants
 constexpr int BW = 16;          // waves per workgroup (1024 threads)
 constexpr int BD = 2;           // prefetch depth
-constexpr int NKIND = 3;        // KG_KIND_LAPLACIAN, _WAVE13PT, _GAUSSBLUR3
+constexpr int NKIND = 5;        // KG_KIND_LAPLACIAN, _WAVE13PT, _GAUSSBLUR3, _JACOBI2, _LIFE2
 constexpr int NTYPE = 2;        // KG_F64, KG_F32
 constexpr int NLPR = 3;         // lanes per row: 64, 32, 16
 constexpr int NVAR = NKIND * NTYPE * NLPR;
@@ -59,6 +62,17 @@ template <> struct Kind<KG_KIND_GAUSSBLUR3> {
     using B = Bounds<Gaussblur3Op>;
     using Sh = Shape<Gaussblur3Op>;
     template <class C> using Op = Gaussblur3Op<C>;
+};
+
+template <> struct Kind<KG_KIND_JACOBI2> {
+    using B = Bounds<Jacobi2Op>;
+    using Sh = Shape<Jacobi2Op>;
+    template <class C> using Op = Jacobi2Op<C>;
+};
+template <> struct Kind<KG_KIND_LIFE2> {
+    using B = Bounds<Life2Op>;
+    using Sh = Shape<Life2Op>;
+    template <class C> using Op = Life2Op<C>;
 };
 
 template <int DT> struct TypeOf;
@@ -90,7 +104,9 @@ constexpr VarInfo var_info()
 #define KG_VAR_LIST(X)                                                       \
     X(0, 0, 0) X(0, 0, 1) X(0, 0, 2) X(0, 1, 0) X(0, 1, 1) X(0, 1, 2)        \
     X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 1, 0) X(1, 1, 1) X(1, 1, 2)        \
-    X(2, 0, 0) X(2, 0, 1) X(2, 0, 2) X(2, 1, 0) X(2, 1, 1) X(2, 1, 2)
+    X(2, 0, 0) X(2, 0, 1) X(2, 0, 2) X(2, 1, 0) X(2, 1, 1) X(2, 1, 2)        \
+    X(3, 0, 0) X(3, 0, 1) X(3, 0, 2) X(3, 1, 0) X(3, 1, 1) X(3, 1, 2)        \
+    X(4, 0, 0) X(4, 0, 1) X(4, 0, 2) X(4, 1, 0) X(4, 1, 1) X(4, 1, 2)
 
 #define KG_VAR_INFO(K, DT, LI) var_info<K, DT, LI>(),
 constexpr VarInfo VARS[NVAR] = {KG_VAR_LIST(KG_VAR_INFO)};
@@ -156,6 +172,15 @@ __device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ 
         op.dst = (S*)g.lev[(ls + 1) & 1];
         op.alpha = (F)g.c[0];
         op.beta = (F)g.c[1];
+    } else if constexpr (K == KG_KIND_JACOBI2) {
+        op.src = (const S*)g.lev[ls & 1];
+        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.c0 = (F)g.c[0];
+        op.c1 = (F)g.c[1];
+        op.c2 = (F)g.c[2];
+    } else if constexpr (K == KG_KIND_LIFE2) {
+        op.src = (const S*)g.lev[ls & 1];
+        op.dst = (S*)g.lev[(ls + 1) & 1];
     } else {
         op.src = (const S*)g.lev[ls & 1];
         op.dst = (S*)g.lev[(ls + 1) & 1];
@@ -205,6 +230,62 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
             __threadfence();
         }
     }
+}
+
+// One tile per workgroup of a single grid (the sequential comparison path:
+// one launch per grid and step, same tile bodies as batch_k).
+template <int K, int DT, int LPR>
+__global__ __launch_bounds__(64 * BW)
+void seq_k(const GridDesc* __restrict__ gp, int ntx, int nty, unsigned long long step)
+{
+    const unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
+    const int bx = (int)(b % (unsigned)ntx);
+    const int by = (int)((b / (unsigned)ntx) % (unsigned)nty);
+    const int bz = (int)(b / ((unsigned)ntx * (unsigned)nty));
+    run_tile<K, DT, LPR>(gp, bx, by, bz, step);
+}
+
+// Uniform [-1, 1) values from a counter-based hash of (seed, element index):
+// the same seed gives the same field whatever the launch shape.
+__device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned long long i)
+{
+    unsigned long long h = (i + 1) * 0x9E3779B97F4A7C15ull ^ seed * 0xD1B54A32D192ED03ull;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    h *= 0x94D049BB133111EBull;
+    h ^= h >> 32;
+    return (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Fill an nx x ny x ns field with row pitch sy (>= nx) and plane pitch
+// sz = sy*ny: element (i,j,k) gets hash_uniform(seed, i + nx*(j + ny*k)), the
+// row padding i >= nx gets 0 (so sums over the padded level are defined).
+template <class S>
+__global__ __launch_bounds__(256)
+void fill_k(S* __restrict__ p, int nx, int ny, int ns, long long sy, long long sz,
+            unsigned long long seed)
+{
+    const long long n = sy * ny * ns;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += stride) {
+        const long long i = e % sy, r = e / sy;
+        const long long j = r % ny, k = r / ny;
+        const S v = i < nx ? (S)hash_uniform(seed, (unsigned long long)(i + nx * (j + ny * k)))
+                           : (S)0;
+        kg_st(p + k * sz + j * sy + i, v);
+    }
+}
+
+template <class S>
+void launch_fill(hipStream_t st, void* p, int nx, int ny, int ns, long long sy, long long sz,
+                 unsigned long long seed)
+{
+    const long long n = sy * ny * ns;
+    const long long blocks = std::min<long long>((n + 255) / 256, 256 * 32);
+    if (n > 0)
+        hipLaunchKernelGGL(fill_k<S>, dim3((unsigned)blocks), dim3(256), 0, st, (S*)p, nx, ny, ns,
+                           sy, sz, seed);
 }
 
 __global__ void state_set_k(BatchState* st, unsigned long long step)
@@ -781,6 +862,103 @@ extern "C" int kg_batch_memory(kg_batch* b, size_t* reserved, size_t* in_use)
     return KG_OK;
 }
 
+// Level l of grid g from seed: hash of (seed, l, element); asynchronous.
+// jacobi2 grids get the same values in both levels: the sweep never writes
+// the boundary, and a relaxation towards a fixed boundary needs the boundary
+// to be the same in the level it reads and the one it writes.
+static void fill_grid(kg_batch* b, const HostGrid& g, unsigned long long seed)
+{
+    for (int l = 0; l < nlev_of(g.spec.kind); l++) {
+        const unsigned long long s =
+            seed * 1000003ull + (g.spec.kind == KG_KIND_JACOBI2 ? 0ull : (unsigned long long)l);
+        if (g.spec.dtype == KG_F64)
+            launch_fill<double>(b->ctx->stream, g.lev[l], g.spec.nx, g.spec.ny, g.spec.ns, g.sy,
+                                g.sz, s);
+        else
+            launch_fill<float>(b->ctx->stream, g.lev[l], g.spec.nx, g.spec.ny, g.spec.ns, g.sy,
+                               g.sz, s);
+    }
+}
+
+extern "C" int kg_batch_fill_grid(kg_batch* b, int id, unsigned long long seed)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive)
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_fill_grid: no grid %d", id);
+    fill_grid(b, b->grids[id], seed);
+    KG_HIP(hipGetLastError());
+    return KG_OK;
+}
+
+extern "C" int kg_batch_fill(kg_batch* b, unsigned long long seed)
+{
+    if (int rc = check_batch(b)) return rc;
+    for (size_t gi = 0; gi < b->grids.size(); gi++)
+        if (b->grids[gi].alive) fill_grid(b, b->grids[gi], seed * 7919ull + gi);
+    KG_HIP(hipGetLastError());
+    KG_HIP(hipStreamSynchronize(b->ctx->stream));
+    return KG_OK;
+}
+
+extern "C" int kg_batch_step_sequential(kg_batch* b, int nsteps)
+{
+    if (int rc = check_batch(b)) return rc;
+    if (nsteps < 0) return kg_set_error(KG_ERR_INVALID, "kg_batch_step_sequential: negative count");
+    if (b->dirty) {
+        if (int rc = build_work(b)) return rc;
+        drop_graphs(b);
+    }
+    kg_register_kernel("batch_sequential", (const void*)seq_k<0, 0, 64>);
+    if (int rc = kg_launch_begin(b->ctx, "batch_sequential")) return rc;
+    for (int s = 0; s < nsteps; s++) {
+        const unsigned long long step = (unsigned long long)(b->steps + s);
+        for (size_t gi = 0; gi < b->grids.size(); gi++) {
+            const HostGrid& g = b->grids[gi];
+            if (!g.alive) continue;
+            const VarInfo& v = VARS[var_id(g.spec.kind, g.spec.dtype, g.li)];
+            const int nxi = g.spec.nx - v.lo - v.hi, nyi = g.spec.ny - v.lo - v.hi;
+            const int nzi = g.spec.ns - v.zlo - v.zhi;
+            if (nxi <= 0 || nyi <= 0 || nzi <= 0) continue;
+            const int ntx = (g.spec.nx + v.txw - 1) / v.txw, nty = (nyi + v.ty - 1) / v.ty;
+            const int nkc = (nzi + g.kc - 1) / g.kc;
+            const GridDesc* gp = b->d_desc + gi;
+            switch (var_id(g.spec.kind, g.spec.dtype, g.li)) {
+#define KG_SEQ_CASE(K, DT, LI)                                                                  \
+    case var_id(K, DT, LI):                                                                     \
+        hipLaunchKernelGGL((seq_k<K, DT, LPRS[LI]>), dim3((unsigned)(ntx * nty * nkc)),         \
+                           dim3(64 * BW), 0, b->ctx->stream, gp, ntx, nty, step);               \
+        break;
+                KG_VAR_LIST(KG_SEQ_CASE)
+#undef KG_SEQ_CASE
+            default: break;
+            }
+        }
+    }
+    KG_HIP(hipGetLastError());
+    // keep the device step counter in line with the host's
+    b->steps += nsteps;
+    hipLaunchKernelGGL(state_set_k, dim3(1), dim3(64), 0, b->ctx->stream, b->d_state,
+                       (unsigned long long)b->steps);
+    return kg_launch_end(b->ctx);
+}
+
+extern "C" int kg_fill_uniform(kg_ctx* ctx, int dtype, size_t n, void* p, unsigned long long seed)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (n && !p) return kg_set_error(KG_ERR_INVALID, "kg_fill_uniform: null buffer");
+    if (n > (size_t)INT32_MAX * 64)
+        return kg_set_error(KG_ERR_INVALID, "kg_fill_uniform: field too large");
+    // as a 1 x n/64-row field when n is a multiple of 64, else n x 1
+    const int nx = (n % 64 == 0) ? 64 : (int)n, ny = (n % 64 == 0) ? (int)(n / 64) : 1;
+    if (n && n % 64 && n > (size_t)INT32_MAX)
+        return kg_set_error(KG_ERR_INVALID, "kg_fill_uniform: field too large");
+    if (dtype == KG_F64) launch_fill<double>(ctx->stream, p, nx, ny, 1, nx, (long long)nx * ny, seed);
+    else if (dtype == KG_F32) launch_fill<float>(ctx->stream, p, nx, ny, 1, nx, (long long)nx * ny, seed);
+    else return kg_set_error(KG_ERR_INVALID, "kg_fill_uniform: dtype %d (F64 or F32)", dtype);
+    KG_HIP(hipGetLastError());
+    return KG_OK;
+}
+
 extern "C" int kg_batch_destroy(kg_batch* b)
 {
     if (!b) return KG_OK;
@@ -795,3 +973,209 @@ extern "C" int kg_batch_destroy(kg_batch* b)
 }
 
 int kg_preload_batch() { return kg_preload_kernel((const void*)batch_k); }
+
+// ================================================================ serving
+//
+// kg_serve_run: a request loop over a batch.  Jobs (grids with a step count)
+// arrive at given times; a job is admitted, in arrival order, as soon as the
+// number of active jobs and the HBM budget allow (its levels are carved from
+// the batch arena and filled on the device); every tick advances every active
+// job by one step with one persistent launch, so jobs that arrive while others
+// run join the next tick and share the device with them (continuous batching).
+// A job leaves when its steps are done (jacobi2 jobs with tol > 0 also when
+// the largest update max |w_new - w_old| falls below tol, checked every
+// KG_SERVE_CHECK steps); its FP64 checksum and optionally its final field are
+// taken before its memory is released.  Mode KG_SERVE_SEQUENTIAL admits one
+// job at a time and steps it with one plain launch per step: the comparison
+// path.
+namespace {
+
+constexpr int KG_SERVE_CHECK = 10;
+
+double now_s(const std::chrono::steady_clock::time_point& t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+size_t job_bytes(const kg_job& j)
+{
+    const int vec = j.dtype == KG_F64 ? 2 : 4, es = j.dtype == KG_F64 ? 8 : 4;
+    const size_t sy = (size_t)((j.nx + vec - 1) / vec * vec);
+    return Arena::up(sy * j.ny * j.ns * es) * nlev_of(j.kind);
+}
+
+struct Active {
+    int job, id;
+    long long done;
+};
+
+}  // namespace
+
+extern "C" int kg_serve_run(kg_ctx* ctx, const kg_job* jobs, int njobs, size_t hbm_budget,
+                            int max_active, int mode, kg_job_result* results,
+                            kg_serve_stats* stats)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (njobs < 0 || (njobs && (!jobs || !results)))
+        return kg_set_error(KG_ERR_INVALID, "kg_serve_run: bad job list");
+    if (mode != KG_SERVE_CONTINUOUS && mode != KG_SERVE_SEQUENTIAL)
+        return kg_set_error(KG_ERR_INVALID, "kg_serve_run: unknown mode %d", mode);
+    if (max_active < 1) max_active = 1;
+    if (mode == KG_SERVE_SEQUENTIAL) max_active = 1;
+    std::vector<int> order(njobs);
+    for (int i = 0; i < njobs; i++) {
+        order[i] = i;
+        kg_grid_spec s{};
+        s.kind = jobs[i].kind;
+        s.dtype = jobs[i].dtype;
+        s.nx = jobs[i].nx;
+        s.ny = jobs[i].ny;
+        s.ns = jobs[i].ns;
+        if (int rc = check_spec(s)) return rc;
+        if (jobs[i].steps < 0) return kg_set_error(KG_ERR_INVALID, "kg_serve_run: negative steps");
+        results[i] = kg_job_result{};
+        results[i].status = KG_OK;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return jobs[a].arrival < jobs[c].arrival; });
+    kg_batch* b = nullptr;
+    if (int rc = kg_batch_create(ctx, &b)) return rc;
+    std::deque<int> pending;
+    std::vector<Active> active;
+    size_t in_use = 0, peak = 0;
+    int next = 0, finished = 0, peak_active = 0;
+    double points = 0;
+    int rc = KG_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+
+    auto finish = [&](const Active& a) -> int {
+        const kg_job& j = jobs[a.job];
+        kg_job_result& r = results[a.job];
+        int lev = 0;
+        if (int e = kg_batch_newest_level(b, a.id, &lev)) return e;
+        const HostGrid& g = b->grids[a.id];
+        double st[5];
+        // checksum over the padded level: the padding columns are never
+        // written after the fill, so equal fields give equal sums
+        if (int e = kg_field_stats(ctx, j.dtype, (size_t)g.sz * j.ns, g.lev[lev], KG_F64, nullptr,
+                                   st))
+            return e;
+        r.checksum[0] = st[0];
+        r.checksum[1] = st[1];
+        if (j.out)
+            if (int e = kg_batch_download(b, a.id, lev, j.out)) return e;
+        if (int e = kg_batch_remove(b, a.id)) return e;   // waits for the device
+        r.finished = now_s(t0);
+        r.steps_run = a.done;
+        in_use -= job_bytes(j);
+        finished++;
+        points += (double)j.nx * j.ny * j.ns * (double)a.done;
+        return KG_OK;
+    };
+
+    while (finished < njobs && !rc) {
+        const double t = now_s(t0);
+        while (next < njobs && jobs[order[next]].arrival <= t) pending.push_back(order[next++]);
+        // admission in arrival order
+        while (!pending.empty() && (int)active.size() < max_active) {
+            const int ji = pending.front();
+            const kg_job& j = jobs[ji];
+            const size_t need = job_bytes(j);
+            if (need > hbm_budget) {   // can never run
+                pending.pop_front();
+                results[ji].status = KG_ERR_NOMEM;
+                results[ji].admitted = results[ji].finished = now_s(t0);
+                finished++;
+                continue;
+            }
+            if (in_use + need > hbm_budget) break;
+            kg_grid_spec s{};
+            s.kind = j.kind;
+            s.dtype = j.dtype;
+            s.nx = j.nx;
+            s.ny = j.ny;
+            s.ns = j.ns;
+            for (int q = 0; q < 6; q++) s.coef[q] = j.coef[q];
+            int id = -1;
+            if ((rc = kg_batch_add(b, &s, &id))) break;
+            fill_grid(b, b->grids[id], j.seed);
+            pending.pop_front();
+            in_use += need;
+            peak = std::max(peak, in_use);
+            results[ji].admitted = now_s(t0);
+            active.push_back(Active{ji, id, 0});
+        }
+        if (rc) break;
+        peak_active = std::max(peak_active, (int)active.size());
+        // jobs with nothing to do leave at once
+        for (size_t a = 0; a < active.size() && !rc;) {
+            if (active[a].done >= jobs[active[a].job].steps) {
+                rc = finish(active[a]);
+                active.erase(active.begin() + a);
+            } else {
+                a++;
+            }
+        }
+        if (rc) break;
+        if (active.empty()) {
+            if (next < njobs && pending.empty()) {
+                const double wait = jobs[order[next]].arrival - now_s(t0);
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            }
+            continue;
+        }
+        // one tick
+        rc = mode == KG_SERVE_SEQUENTIAL ? kg_batch_step_sequential(b, 1) : kg_batch_step(b, 1, 0);
+        if (rc) break;
+        for (auto& a : active) a.done++;
+        for (size_t a = 0; a < active.size() && !rc;) {
+            const kg_job& j = jobs[active[a].job];
+            bool done = active[a].done >= j.steps;
+            if (!done && j.kind == KG_KIND_JACOBI2 && j.tol > 0 && active[a].done % KG_SERVE_CHECK == 0) {
+                const HostGrid& g = b->grids[active[a].id];
+                int lev = 0;
+                if ((rc = kg_batch_newest_level(b, active[a].id, &lev))) break;
+                double st[5];
+                if ((rc = kg_field_stats(ctx, j.dtype, (size_t)g.sz * j.ns, g.lev[lev], j.dtype,
+                                         g.lev[lev ^ 1], st)))
+                    break;
+                done = st[4] < j.tol;
+            }
+            if (done) {
+                rc = finish(active[a]);
+                active.erase(active.begin() + a);
+            } else {
+                a++;
+            }
+        }
+    }
+    const double wall = now_s(t0);
+    if (rc) {
+        kg_batch_destroy(b);
+        return rc;
+    }
+    kg_batch_destroy(b);
+    if (stats) {
+        std::vector<double> lat;
+        for (int i = 0; i < njobs; i++)
+            if (results[i].status == KG_OK) lat.push_back(results[i].finished - jobs[i].arrival);
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) {
+            if (lat.empty()) return 0.0;
+            const double pos = p * (lat.size() - 1);
+            const size_t lo = (size_t)pos;
+            const size_t hi = std::min(lo + 1, lat.size() - 1);
+            return lat[lo] + (pos - lo) * (lat[hi] - lat[lo]);
+        };
+        double mean = 0;
+        for (double v : lat) mean += v;
+        stats->wall = wall;
+        stats->points = points;
+        stats->p50 = pct(0.5);
+        stats->p99 = pct(0.99);
+        stats->mean_latency = lat.empty() ? 0 : mean / lat.size();
+        stats->peak_active = peak_active;
+        stats->peak_bytes = peak;
+    }
+    return KG_OK;
+}

@@ -94,11 +94,64 @@ struct Gaussblur3Op {
     }
 };
 
+// jacobi/jacobi.F90:65-67 applied to every xy-plane (a 2D nx x ny problem
+// is a grid with ns = 1): xy-interior [1,n-1), all planes.  Term order as the
+// FP64 2D entry point kg_jacobi.
+template <class C>
+struct Jacobi2Op {
+    using S = typename C::S;
+    using F = typename C::F;
+    const S* src;
+    const S* side[1];
+    S* dst;
+    F c0, c1, c2;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        F r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++)
+            r[v] = c0 * p.at(v, 0, 0, 0) +
+                   c1 * (p.at(v, -1, 0, 0) + p.at(v, 0, -1, 0) + p.at(v, 1, 0, 0) + p.at(v, 0, 1, 0)) +
+                   c2 * (p.at(v, -1, -1, 0) + p.at(v, -1, 1, 0) + p.at(v, 1, -1, 0) + p.at(v, 1, 1, 0));
+        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
+// gameoflife/gameoflife.c:82-91 applied to every xy-plane: xy-interior
+// [1,n-1), all planes.  Same expression as the FP64 2D entry point.
+template <class C>
+struct Life2Op {
+    using S = typename C::S;
+    using F = typename C::F;
+    const S* src;
+    const S* side[1];
+    S* dst;
+    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+                                        const bool (&xm)[C::VEC]) const
+    {
+        F r[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) {
+#define U(di, dj) p.at(v, di, dj, 0)
+            const F Cbig = (F)100000000000000000000.;
+            const F L = U(-1, -1) + U(0, -1) + U(1, -1) +
+                        U(-1, 0) + U(1, 0) +
+                        U(-1, 1) + U(0, 1) + U(1, 1);
+            r[v] = (F)1. / ((F)1. + (U(0, 0) + L - (F)3.) * (L - (F)3.) * Cbig);
+#undef U
+        }
+        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+    }
+};
+
 // Interior bounds of each op: xy [LO, n-HI), z [ZLO, ns-ZHI).
 template <template <class> class Op> struct Bounds;
 template <> struct Bounds<LaplacianOp> { static constexpr int LO = 1, HI = 1, ZLO = 1, ZHI = 1; };
 template <> struct Bounds<Wave13ptOp> { static constexpr int LO = 2, HI = 2, ZLO = 2, ZHI = 2; };
 template <> struct Bounds<Gaussblur3Op> { static constexpr int LO = 2, HI = 2, ZLO = 0, ZHI = 0; };
+template <> struct Bounds<Jacobi2Op> { static constexpr int LO = 1, HI = 1, ZLO = 0, ZHI = 0; };
+template <> struct Bounds<Life2Op> { static constexpr int LO = 1, HI = 1, ZLO = 0, ZHI = 0; };
 
 }  // namespace kgops
 
@@ -110,6 +163,8 @@ template <template <class> class Op> struct Shape;
 template <> struct Shape<LaplacianOp> { static constexpr int H = 1, LL = 0, LR = 0, ZL = 1, ZR = 1, NAUX = 0; };
 template <> struct Shape<Wave13ptOp> { static constexpr int H = 2, LL = 0, LR = 0, ZL = 2, ZR = 2, NAUX = 1; };
 template <> struct Shape<Gaussblur3Op> { static constexpr int H = 2, LL = 0, LR = 0, ZL = 0, ZR = 0, NAUX = 0; };
+template <> struct Shape<Jacobi2Op> { static constexpr int H = 1, LL = 0, LR = 0, ZL = 0, ZR = 0, NAUX = 0; };
+template <> struct Shape<Life2Op> { static constexpr int H = 1, LL = 0, LR = 0, ZL = 0, ZR = 0, NAUX = 0; };
 
 // Engine configuration of op Op with storage type T, VEC points per lane, W
 // waves, prefetch depth D, KC planes per single-grid tile, LPR lanes per row.

@@ -46,6 +46,7 @@ SIGNATURES = {
     "kg_gaussblur3_f64": [_i, _i, _i, _d, _d, _d, _d, _d, _d, _dp, _dp],
     "kg_gaussblur3_f32": [_i, _i, _i, _f, _f, _f, _f, _f, _f, _dp, _dp],
     "kg_field_stats": [_i, C.c_size_t, _dp, _i, _dp, C.POINTER(_d)],
+    "kg_fill_uniform": [_i, C.c_size_t, _dp, C.c_ulonglong],
     "kg_matmul": [_i, _i, _i, _dp, _dp, _dp],
     "kg_stream_copy": [_dp, _dp, C.c_size_t],
 }
@@ -235,9 +236,10 @@ class Context:
 
 
 # --------------------------------------------------------------- batches
-KIND_LAPLACIAN, KIND_WAVE13PT, KIND_GAUSSBLUR3 = 0, 1, 2
-KINDS = {"laplacian": KIND_LAPLACIAN, "wave13pt": KIND_WAVE13PT, "gaussblur3": KIND_GAUSSBLUR3}
-NLEV = {KIND_LAPLACIAN: 2, KIND_WAVE13PT: 3, KIND_GAUSSBLUR3: 2}
+KIND_LAPLACIAN, KIND_WAVE13PT, KIND_GAUSSBLUR3, KIND_JACOBI2, KIND_LIFE2 = 0, 1, 2, 3, 4
+KINDS = {"laplacian": KIND_LAPLACIAN, "wave13pt": KIND_WAVE13PT, "gaussblur3": KIND_GAUSSBLUR3,
+         "jacobi2": KIND_JACOBI2, "life2": KIND_LIFE2}
+NLEV = {KIND_LAPLACIAN: 2, KIND_WAVE13PT: 3, KIND_GAUSSBLUR3: 2, KIND_JACOBI2: 2, KIND_LIFE2: 2}
 
 
 class GridSpec(C.Structure):
@@ -260,8 +262,11 @@ _BATCH_SIGS = {
     "kg_batch_grid_shape": [_dp, _i, C.POINTER(_i), C.POINTER(_i)],
     "kg_batch_set_shape": [_dp, _i, _i, _i],
     "kg_batch_memory": [_dp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "kg_batch_fill": [_dp, C.c_ulonglong],
+    "kg_batch_fill_grid": [_dp, _i, C.c_ulonglong],
+    "kg_batch_step_sequential": [_dp, _i],
 }
-EXPORTS += list(_BATCH_SIGS)
+EXPORTS += list(_BATCH_SIGS) + ["kg_serve_run"]
 
 
 def _declare_batch(lib):
@@ -372,8 +377,78 @@ class Batch:
         _check(self.lib, self.lib.kg_batch_set_shape(self.ptr, gid, lanes_per_row,
                                                      planes_per_tile), "kg_batch_set_shape")
 
+    def fill(self, seed: int) -> None:
+        _check(self.lib, self.lib.kg_batch_fill(self.ptr, seed), "kg_batch_fill")
+
+    def step_sequential(self, n: int = 1) -> None:
+        _check(self.lib, self.lib.kg_batch_step_sequential(self.ptr, n), "kg_batch_step_sequential")
+
     def memory(self) -> tuple[int, int]:
         r, u = C.c_size_t(), C.c_size_t()
         _check(self.lib, self.lib.kg_batch_memory(self.ptr, C.byref(r), C.byref(u)),
                "kg_batch_memory")
         return r.value, u.value
+
+
+# ---------------------------------------------------------------- serving
+SERVE_CONTINUOUS, SERVE_SEQUENTIAL = 0, 1
+
+
+class Job(C.Structure):
+    _fields_ = [("kind", _i), ("dtype", _i), ("nx", _i), ("ny", _i), ("ns", _i), ("steps", _i),
+                ("tol", _d), ("coef", _d * 6), ("arrival", _d), ("seed", C.c_ulonglong),
+                ("out", _dp)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("admitted", _d), ("finished", _d), ("steps_run", C.c_longlong),
+                ("checksum", _d * 2), ("status", _i)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall", _d), ("points", _d), ("p50", _d), ("p99", _d), ("mean_latency", _d),
+                ("peak_active", _i), ("peak_bytes", C.c_size_t)]
+
+
+def serve(ctx: Context, jobs: list[dict], hbm_budget: int, max_active: int = 64,
+          mode: int = SERVE_CONTINUOUS, outputs: bool = False):
+    """Run kg_serve_run over jobs (dicts: kind, dtype, dims, steps, coef,
+    arrival, seed, tol).  Returns (results, stats, outs) where outs[i] is the
+    final field (ns, ny, nx) when outputs is set."""
+    lib = ctx.lib
+    fn = lib.kg_serve_run
+    fn.argtypes = [_dp, C.POINTER(Job), _i, C.c_size_t, _i, _i, C.POINTER(JobResult),
+                   C.POINTER(ServeStats)]
+    fn.restype = _i
+    n = len(jobs)
+    arr = (Job * max(n, 1))()
+    outs = []
+    for i, j in enumerate(jobs):
+        a = arr[i]
+        a.kind = KINDS[j["kind"]]
+        a.dtype = j["dtype"]
+        a.nx, a.ny, a.ns = j["dims"]
+        a.steps = j["steps"]
+        a.tol = j.get("tol", 0.0)
+        for q, c in enumerate(j.get("coef", ())):
+            a.coef[q] = c
+        a.arrival = j.get("arrival", 0.0)
+        a.seed = j.get("seed", i + 1)
+        if outputs:
+            nx, ny, ns = j["dims"]
+            o = np.empty((ns, ny, nx), np.float64 if j["dtype"] == F64 else np.float32)
+            outs.append(o)
+            a.out = o.ctypes.data
+        else:
+            outs.append(None)
+            a.out = None
+    res = (JobResult * max(n, 1))()
+    st = ServeStats()
+    _check(lib, fn(ctx.ptr, arr, n, hbm_budget, max_active, mode, res, C.byref(st)),
+           "kg_serve_run")
+    results = [{"admitted": res[i].admitted, "finished": res[i].finished,
+                "latency": res[i].finished - jobs[i].get("arrival", 0.0),
+                "steps_run": res[i].steps_run, "checksum": (res[i].checksum[0], res[i].checksum[1]),
+                "status": res[i].status} for i in range(n)]
+    stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
+    return results, stats, outs

@@ -194,3 +194,26 @@ def test_batch_rejects_bad_specs(kgmod, batch):
         batch.add("laplacian", 0, (0, 8, 8), (0.1, 0.2))
     with pytest.raises(kgmod.KGError):
         batch.remove(12345)
+
+
+def test_batch_fill_and_sequential_path(gpu_ctx, kgmod):
+    """kg_batch_fill is deterministic in the seed; the per-grid sequential
+    launches give the persistent kernel's results."""
+    rng = np.random.default_rng(6)
+    specs = _mixed_specs(rng, 9, 4, 60)
+    outs = []
+    for mode in ("batch", "sequential"):
+        b = kgmod.Batch(gpu_ctx)
+        ids = [b.add(k, d, dims, COEFS[k]) for k, d, dims in specs]
+        b.fill(1234)
+        first = [b.download(g, 0) for g in ids]
+        if mode == "batch":
+            b.step(5, graph=True)
+        else:
+            b.step_sequential(5)
+        outs.append((first, [b.download(g) for g in ids]))
+        b.close()
+    for (k, d, dims), f0, f1, o0, o1 in zip(specs, outs[0][0], outs[1][0], outs[0][1], outs[1][1]):
+        assert f0.tobytes() == f1.tobytes()
+        assert np.all(np.abs(f0) <= 1) and np.std(f0) > 0.3
+        assert o0.tobytes() == o1.tobytes(), (k, d, dims)
