@@ -175,6 +175,10 @@ int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0, double s1,
 int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, float s1, float s2,
                       float s4, float s5, float s8, const float* w0, float* w1);
 
+/* n values from src (src_dtype) to dst (dst_dtype), rounded to nearest even;
+ * asynchronous on the context stream. */
+int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, const void* src, size_t n);
+
 /* Fills n values (KG_F64 or KG_F32) at p with uniform [-1, 1) values from a
  * counter-based hash of (seed, index); asynchronous on the context stream. */
 int kg_fill_uniform(kg_ctx* ctx, int dtype, size_t n, void* p, unsigned long long seed);
@@ -243,9 +247,10 @@ int kg_batch_step(kg_batch* b, int nsteps, int use_graph);
 int kg_batch_info(kg_batch* b, double* points_per_step, double* bytes_per_step,
                   int* work_items, int* workgroups);
 /* Autotuner: for every size class present (kind, type, nx and ny bucket),
- * times each tile shape (16/32/64 lanes per row x 8/16/32/64 planes per tile)
- * over reps launches on the batch's own grids and keeps the fastest.  Grid
- * contents are preserved. */
+ * times each tile shape -- the plane engine with 16/32/64 lanes per row, or
+ * the flat body (lanes_per_row 0: points swept from cache, for small grids) --
+ * with 8/16/32/64 planes per tile, over reps launches on the batch's own
+ * grids, and keeps the fastest.  Grid contents are preserved. */
 int kg_batch_tune(kg_batch* b, int reps, int* classes_tuned);
 int kg_batch_grid_shape(kg_batch* b, int id, int* lanes_per_row, int* planes_per_tile);
 /* Forces the tile shape of the size class of grid id. */

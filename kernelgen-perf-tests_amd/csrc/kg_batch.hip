@@ -37,13 +37,30 @@ namespace {
 using namespace kgops;
 
 // ------------------------------------------------------------ variants
-constexpr int BW = 16;          // waves per workgroup (1024 threads)
-constexpr int BD = 2;           // prefetch depth
+#ifndef KG_BATCH_WAVES
+#define KG_BATCH_WAVES 16
+#endif
+#ifndef KG_BATCH_DEPTH
+#define KG_BATCH_DEPTH 2
+#endif
+#ifndef KG_BATCH_WPE
+#define KG_BATCH_WPE 4
+#endif
+#ifndef KG_BATCH_VEC64
+#define KG_BATCH_VEC64 2
+#endif
+#ifndef KG_BATCH_VEC32
+#define KG_BATCH_VEC32 4
+#endif
+constexpr int BW = KG_BATCH_WAVES;   // waves per workgroup
+constexpr int BD = KG_BATCH_DEPTH;   // prefetch depth (planes)
 constexpr int NKIND = 5;        // KG_KIND_LAPLACIAN, _WAVE13PT, _GAUSSBLUR3, _JACOBI2, _LIFE2
 constexpr int NTYPE = 2;        // KG_F64, KG_F32
-constexpr int NLPR = 3;         // lanes per row: 64, 32, 16
-constexpr int NVAR = NKIND * NTYPE * NLPR;
-constexpr int LPRS[NLPR] = {64, 32, 16};
+constexpr int NLPR = 3;         // plane-engine shapes: lanes per row 64, 32, 16
+constexpr int FLAT = NLPR;      // shape index of the flat (cache-served) body
+constexpr int NSHAPE = NLPR + 1;
+constexpr int NVAR = NKIND * NTYPE * NSHAPE;
+constexpr int LPRS[NSHAPE] = {64, 32, 16, 0};   // 0: flat
 constexpr int KCS[] = {8, 16, 32, 64};
 constexpr int NKC = sizeof(KCS) / sizeof(KCS[0]);
 
@@ -76,15 +93,44 @@ template <> struct Kind<KG_KIND_LIFE2> {
 };
 
 template <int DT> struct TypeOf;
-template <> struct TypeOf<KG_F64> { using T = kgp::F64; static constexpr int VEC = 2; };
-template <> struct TypeOf<KG_F32> { using T = kgp::F32; static constexpr int VEC = 4; };
+template <> struct TypeOf<KG_F64> { using T = kgp::F64; static constexpr int VEC = KG_BATCH_VEC64; };
+template <> struct TypeOf<KG_F32> { using T = kgp::F32; static constexpr int VEC = KG_BATCH_VEC32; };
+
+// Prefetch depth (planes) per kind and type.  With one 16-wave workgroup
+// per CU the bytes in flight per CU are about depth x tile rows x row bytes,
+// and the achieved bandwidth follows them; each body takes the deepest
+// prefetch that fits in 128 VGPRs without spilling (the ring holds
+// ZL + ZHI + depth + 1 planes of VEC + 1 values).
+#ifndef KG_BATCH_DEEP
+#define KG_BATCH_DEEP 0
+#endif
+constexpr int depth_of(int kind, int dt)
+{
+    if (KG_BATCH_DEEP > 0) return KG_BATCH_DEEP;
+    if (kind == KG_KIND_WAVE13PT) return 3;
+    if (kind == KG_KIND_GAUSSBLUR3) return dt == KG_F64 ? 2 : 3;
+    return dt == KG_F64 ? 3 : 4;
+}
+
+// Points per lane: 16-byte lanes, except wave13pt (13 neighbours, 3 levels),
+// whose 16-byte FP64/FP32 bodies do not fit 128 VGPRs with a useful prefetch.
+constexpr int vec_of(int kind, int dt)
+{
+    if (kind == KG_KIND_WAVE13PT) return dt == KG_F64 ? 1 : 2;
+    return dt == KG_F64 ? TypeOf<KG_F64>::VEC : TypeOf<KG_F32>::VEC;
+}
 
 template <int K, int DT, int LPR>
-using VarCfg = kgp::Cfg<TypeOf<DT>::VEC, BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind<K>::Sh::LR,
-                        Kind<K>::Sh::ZL, Kind<K>::Sh::ZR, BD, Kind<K>::Sh::NAUX, 32,
+using VarCfg = kgp::Cfg<vec_of(K, DT), BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind<K>::Sh::LR,
+                        Kind<K>::Sh::ZL, Kind<K>::Sh::ZR, depth_of(K, DT), Kind<K>::Sh::NAUX, 32,
                         typename TypeOf<DT>::T, LPR>;
 
-constexpr int var_id(int kind, int dt, int li) { return (kind * NTYPE + dt) * NLPR + li; }
+constexpr int var_id(int kind, int dt, int li) { return (kind * NTYPE + dt) * NSHAPE + li; }
+
+// Flat body configuration: one point per lane, operands from global memory.
+template <int K, int DT>
+using FlatCfg = kgp::Cfg<1, BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind<K>::Sh::LR, Kind<K>::Sh::ZL,
+                         Kind<K>::Sh::ZR, BD, Kind<K>::Sh::NAUX, 32, typename TypeOf<DT>::T, 64>;
 
 // Shape facts of every variant, for the host (tile counts) and the LDS size.
 struct VarInfo {
@@ -95,10 +141,15 @@ struct VarInfo {
 template <int K, int DT, int LI>
 constexpr VarInfo var_info()
 {
-    using C = VarCfg<K, DT, LPRS[LI]>;
     using B = typename Kind<K>::B;
-    return VarInfo{C::TXW, C::TY, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC, (int)sizeof(typename C::S),
-                   C::LDS_BYTES};
+    if constexpr (LI == FLAT) {
+        using C = FlatCfg<K, DT>;
+        return VarInfo{0, 0, B::LO, B::HI, B::ZLO, B::ZHI, 1, (int)sizeof(typename C::S), 0};
+    } else {
+        using C = VarCfg<K, DT, LPRS[LI]>;
+        return VarInfo{C::TXW, C::TY, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC,
+                       (int)sizeof(typename C::S), C::LDS_BYTES};
+    }
 }
 
 #define KG_VAR_LIST(X)                                                       \
@@ -108,8 +159,12 @@ constexpr VarInfo var_info()
     X(3, 0, 0) X(3, 0, 1) X(3, 0, 2) X(3, 1, 0) X(3, 1, 1) X(3, 1, 2)        \
     X(4, 0, 0) X(4, 0, 1) X(4, 0, 2) X(4, 1, 0) X(4, 1, 1) X(4, 1, 2)
 
-#define KG_VAR_INFO(K, DT, LI) var_info<K, DT, LI>(),
-constexpr VarInfo VARS[NVAR] = {KG_VAR_LIST(KG_VAR_INFO)};
+#define KG_FLAT_LIST(X) \
+    X(0, 0) X(0, 1) X(1, 0) X(1, 1) X(2, 0) X(2, 1) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
+
+#define KG_VAR_INFO4(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
+                            var_info<K, DT, FLAT>(),
+constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO4)};
 
 constexpr size_t max_lds()
 {
@@ -136,9 +191,9 @@ struct Item {
 };
 
 struct BatchState {
-    unsigned long long step;   // steps completed by the batch
-    unsigned counter;          // next work item
-    unsigned done;             // workgroups finished this launch
+    unsigned long long step;          // steps completed by the batch
+    unsigned counter[KG_NUM_XCD];     // next item of each XCD's part of the list
+    unsigned done;                    // workgroups finished this launch
 };
 
 // Not inlined: each variant then gets its own register allocation and the
@@ -148,16 +203,30 @@ struct BatchState {
 // scope so that the non-inlined tile bodies address it as LDS.
 __shared__ __attribute__((aligned(16))) char batch_lds[BATCH_LDS];
 
-template <int K, int DT, int LPR>
-__device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ gp, int bx, int by,
-                                                   int bz, unsigned long long step)
+// Arguments of a non-inlined device function arrive in vector registers;
+// these mark workgroup-uniform ones as such, so that the descriptor is read
+// with scalar loads into scalar registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uni(unsigned long long v)
 {
-    const GridDesc& g = *gp;
-    using C = VarCfg<K, DT, LPR>;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ const GridDesc* uni(const GridDesc* p)
+{
+    return (const GridDesc*)uni((unsigned long long)(uintptr_t)p);
+}
+
+// The op of grid g at batch step `step`: levels chosen from the grid's local
+// step, coefficients converted to the arithmetic type.
+template <int K, class C>
+__device__ __forceinline__ typename Kind<K>::template Op<C> make_op(const GridDesc& g,
+                                                                      unsigned long long step)
+{
     using Op = typename Kind<K>::template Op<C>;
     using S = typename C::S;
     using F = typename C::F;
-    using B = typename Kind<K>::B;
     const long long ls = (long long)step - g.step0;
     Op op{};
     if constexpr (K == KG_KIND_WAVE13PT) {
@@ -192,23 +261,88 @@ __device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ 
         op.s5 = (F)g.c[5];
         op.s8 = (F)g.c[6];
     }
+    return op;
+}
+
+template <int K, int DT, int LPR>
+__device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ gp_, int bx_,
+                                                   int by_, int bz_, unsigned long long step_)
+{
+    const GridDesc& g = *uni(gp_);
+    const int bx = uni(bx_), by = uni(by_), bz = uni(bz_);
+    const unsigned long long step = uni(step_);
+    using C = VarCfg<K, DT, LPR>;
+    using Op = typename Kind<K>::template Op<C>;
+    using F = typename C::F;
+    using B = typename Kind<K>::B;
+    const Op op = make_op<K, C>(g, step);
     kgp::plane_tile<C, B::LO, B::HI, B::ZLO, B::ZHI, Op>(
         (F(*)[C::ROWS][C::LW])batch_lds, g.nx, g.ny, g.ns, g.sy, g.sz, bx, by, bz, g.kc, op);
 }
 
-__global__ __launch_bounds__(64 * BW)
+// Flat body: the workgroup sweeps the interior points of planes
+// ZLO + bz*kc .. +kc, one point per lane per round, x fastest; operands come
+// from global memory, where a small grid's planes stay cached (L1/L2).
+template <int K, int DT>
+__device__ __attribute__((noinline)) void run_flat(const GridDesc* __restrict__ gp_, int bz_,
+                                                   unsigned long long step_)
+{
+    const GridDesc& g = *uni(gp_);
+    const int bz = uni(bz_);
+    const unsigned long long step = uni(step_);
+    using C = FlatCfg<K, DT>;
+    using B = typename Kind<K>::B;
+    const auto op = make_op<K, C>(g, step);
+    const unsigned nxi = (unsigned)(g.nx - B::LO - B::HI), nyi = (unsigned)(g.ny - B::LO - B::HI);
+    const int k0 = B::ZLO + bz * g.kc;
+    const int k1 = min(k0 + g.kc, g.ns - B::ZHI);
+    const unsigned plane = nxi * nyi;
+    const unsigned npts = plane * (unsigned)(k1 - k0);
+    FlatView<C> v;
+    v.src = kgp::as_global(op.src);
+    v.aux[0] = C::NAUX ? kgp::as_global(op.side[0]) : nullptr;
+    v.sy = g.sy;
+    v.sz = g.sz;
+    const bool xm[1] = {true};
+    for (unsigned p = threadIdx.x; p < npts; p += blockDim.x) {
+        const unsigned kk = p / plane, r = p - kk * plane;
+        const unsigned jj = r / nxi, ii = r - jj * nxi;
+        const int i = B::LO + (int)ii;
+        const long long ob = (long long)(k0 + (int)kk) * g.sz + (long long)(B::LO + (int)jj) * g.sy;
+        v.o = ob + i;
+        op.run(v, ob, i, xm);
+    }
+}
+
+// 4 waves per SIMD: one 16-wave or two 8-wave workgroups per CU, 128 VGPRs.
+__global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
 void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items, int nitems,
              BatchState* st)
 {
     __shared__ int next;
     const unsigned long long step = __hip_atomic_load(&st->step, __ATOMIC_RELAXED,
                                                       __HIP_MEMORY_SCOPE_AGENT);
+    // The list is cut into KG_NUM_XCD contiguous parts.  Workgroups are
+    // dispatched round-robin over the XCDs, so workgroup b runs on XCD b % 8
+    // and takes items from part b % 8 first: neighbouring tiles, which share
+    // halo rows and planes, then run on one XCD and share its L2.  A
+    // workgroup whose part is empty moves on to the next part.
+    const int home = (int)(blockIdx.x % KG_NUM_XCD);
+    int part = home, lo = 0, hi = 0, parts_left = KG_NUM_XCD;
+    lo = (int)((long long)part * nitems / KG_NUM_XCD);
+    hi = (int)((long long)(part + 1) * nitems / KG_NUM_XCD);
     for (;;) {
-        if (threadIdx.x == 0) next = (int)atomicAdd(&st->counter, 1u);
+        if (threadIdx.x == 0) next = lo + (int)atomicAdd(&st->counter[part], 1u);
         __syncthreads();
         const int i = next;
         __syncthreads();
-        if (i >= nitems) break;
+        if (i >= hi) {
+            if (--parts_left == 0) break;
+            part = (part + 1) % KG_NUM_XCD;
+            lo = (int)((long long)part * nitems / KG_NUM_XCD);
+            hi = (int)((long long)(part + 1) * nitems / KG_NUM_XCD);
+            continue;
+        }
         const Item it = items[i];
         const GridDesc* g = grids + it.grid;
         switch (g->var) {
@@ -216,6 +350,10 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
     case var_id(K, DT, LI): run_tile<K, DT, LPRS[LI]>(g, it.bx, it.by, it.bz, step); break;
             KG_VAR_LIST(KG_VAR_CASE)
 #undef KG_VAR_CASE
+#define KG_FLAT_CASE(K, DT) \
+    case var_id(K, DT, FLAT): run_flat<K, DT>(g, it.bz, step); break;
+            KG_FLAT_LIST(KG_FLAT_CASE)
+#undef KG_FLAT_CASE
         default: break;
         }
         __syncthreads();   // the next tile restages LDS
@@ -224,7 +362,7 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
         __threadfence();
         if (atomicAdd(&st->done, 1u) == gridDim.x - 1) {
             // every workgroup has taken its last item and read `step`
-            atomicExch(&st->counter, 0u);
+            for (int x = 0; x < KG_NUM_XCD; x++) atomicExch(&st->counter[x], 0u);
             atomicExch(&st->done, 0u);
             atomicExch(&st->step, step + 1);
             __threadfence();
@@ -235,7 +373,7 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
 // One tile per workgroup of a single grid (the sequential comparison path:
 // one launch per grid and step, same tile bodies as batch_k).
 template <int K, int DT, int LPR>
-__global__ __launch_bounds__(64 * BW)
+__global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
 void seq_k(const GridDesc* __restrict__ gp, int ntx, int nty, unsigned long long step)
 {
     const unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
@@ -243,6 +381,13 @@ void seq_k(const GridDesc* __restrict__ gp, int ntx, int nty, unsigned long long
     const int by = (int)((b / (unsigned)ntx) % (unsigned)nty);
     const int bz = (int)(b / ((unsigned)ntx * (unsigned)nty));
     run_tile<K, DT, LPR>(gp, bx, by, bz, step);
+}
+
+template <int K, int DT>
+__global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
+void seq_flat_k(const GridDesc* __restrict__ gp, unsigned long long step)
+{
+    run_flat<K, DT>(gp, (int)blockIdx.x, step);
 }
 
 // Uniform [-1, 1) values from a counter-based hash of (seed, element index):
@@ -292,7 +437,7 @@ __global__ void state_set_k(BatchState* st, unsigned long long step)
 {
     if (threadIdx.x == 0) {
         atomicExch(&st->step, step);
-        atomicExch(&st->counter, 0u);
+        for (int x = 0; x < KG_NUM_XCD; x++) atomicExch(&st->counter[x], 0u);
         atomicExch(&st->done, 0u);
     }
 }
@@ -391,9 +536,10 @@ int size_class(const kg_grid_spec& s)
 
 void default_shape(const kg_grid_spec& s, int& li, int& kc)
 {
-    const int vec = s.dtype == KG_F64 ? 2 : 4;
+    const int vec = s.dtype == KG_F64 ? TypeOf<KG_F64>::VEC : TypeOf<KG_F32>::VEC;
     li = 0;
-    if (s.nx <= 16 * vec) li = 2;
+    if ((long long)s.nx * s.ny <= 1024) li = FLAT;
+    else if (s.nx <= 16 * vec) li = 2;
     else if (s.nx <= 32 * vec) li = 1;
     kc = s.ns <= 24 ? 8 : s.ns <= 48 ? 16 : 32;
 }
@@ -433,6 +579,10 @@ void cut_tiles(const HostGrid& g, int gi, int li, int kc, std::vector<Item>& out
     const int nx = g.spec.nx, ny = g.spec.ny, ns = g.spec.ns;
     const int nxi = nx - v.lo - v.hi, nyi = ny - v.lo - v.hi, nzi = ns - v.zlo - v.zhi;
     if (nxi <= 0 || nyi <= 0 || nzi <= 0) return;
+    if (li == FLAT) {
+        for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++) out.push_back(Item{gi, 0, 0, bz});
+        return;
+    }
     const int ntx = (nx + v.txw - 1) / v.txw;
     const int nty = (nyi + v.ty - 1) / v.ty;
     const int nkc = (nzi + kc - 1) / kc;
@@ -579,7 +729,8 @@ int check_spec(const kg_grid_spec& s)
         return kg_set_error(KG_ERR_INVALID, "kg_batch: dtype %d not batched (F64 or F32)", s.dtype);
     if (s.nx < 1 || s.ny < 1 || s.ns < 1 || (long long)s.nx * s.ny > (1LL << 31))
         return kg_set_error(KG_ERR_INVALID, "kg_batch: bad grid %dx%dx%d", s.nx, s.ny, s.ns);
-    return KG_OK;
+    // padded rows (up to 3 extra values) of 8 or 4 bytes
+    return kg_check_plane_bytes((long long)(s.nx + 3) * s.ny, s.dtype == KG_F64 ? 8 : 4);
 }
 
 }  // namespace
@@ -618,7 +769,7 @@ extern "C" int kg_batch_add(kg_batch* b, const kg_grid_spec* spec, int* id)
     HostGrid g{};
     g.spec = *spec;
     g.alive = true;
-    const int vec = spec->dtype == KG_F64 ? 2 : 4;
+    const int vec = spec->dtype == KG_F64 ? TypeOf<KG_F64>::VEC : TypeOf<KG_F32>::VEC;
     const int es = spec->dtype == KG_F64 ? 8 : 4;
     g.sy = (spec->nx + vec - 1) / vec * vec;
     g.sz = g.sy * spec->ny;
@@ -780,7 +931,7 @@ extern "C" int kg_batch_tune(kg_batch* b, int reps, int* classes_tuned)
         const int c = cls.first;
         float best = 1e30f;
         std::pair<int, int> pick{-1, -1};
-        for (int li = 0; li < NLPR && !rc; li++)
+        for (int li = 0; li < NSHAPE && !rc; li++)
             for (int ki = 0; ki < NKC && !rc; ki++) {
                 if ((rc = build_work(b, c, li, KCS[ki]))) break;
                 b->d_state = scratch;
@@ -840,11 +991,11 @@ extern "C" int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int pl
     if (id < 0 || id >= (int)b->grids.size() || !b->grids[id].alive)
         return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: no grid %d", id);
     int li = -1;
-    for (int q = 0; q < NLPR; q++)
+    for (int q = 0; q < NSHAPE; q++)
         if (LPRS[q] == lanes_per_row) li = q;
     if (li < 0 || planes_per_tile < 1)
-        return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: lanes per row must be 16, 32 "
-                                            "or 64 and planes per tile positive");
+        return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: lanes per row must be 16, 32, "
+                                            "64 or 0 (flat) and planes per tile positive");
     b->tuned[size_class(b->grids[id].spec)] = {li, planes_per_tile};
     b->dirty = true;
     drop_graphs(b);
@@ -919,6 +1070,21 @@ extern "C" int kg_batch_step_sequential(kg_batch* b, int nsteps)
             const int nxi = g.spec.nx - v.lo - v.hi, nyi = g.spec.ny - v.lo - v.hi;
             const int nzi = g.spec.ns - v.zlo - v.zhi;
             if (nxi <= 0 || nyi <= 0 || nzi <= 0) continue;
+            if (g.li == FLAT) {
+                const GridDesc* gp = b->d_desc + gi;
+                const unsigned nkc = (unsigned)((nzi + g.kc - 1) / g.kc);
+                switch (g.spec.kind * NTYPE + g.spec.dtype) {
+#define KG_SEQ_FLAT(K, DT)                                                                      \
+    case K * NTYPE + DT:                                                                        \
+        hipLaunchKernelGGL((seq_flat_k<K, DT>), dim3(nkc), dim3(64 * BW), 0, b->ctx->stream, gp, \
+                           step);                                                               \
+        break;
+                    KG_FLAT_LIST(KG_SEQ_FLAT)
+#undef KG_SEQ_FLAT
+                default: break;
+                }
+                continue;
+            }
             const int ntx = (g.spec.nx + v.txw - 1) / v.txw, nty = (nyi + v.ty - 1) / v.ty;
             const int nkc = (nzi + g.kc - 1) / g.kc;
             const GridDesc* gp = b->d_desc + gi;
@@ -999,7 +1165,8 @@ double now_s(const std::chrono::steady_clock::time_point& t0)
 
 size_t job_bytes(const kg_job& j)
 {
-    const int vec = j.dtype == KG_F64 ? 2 : 4, es = j.dtype == KG_F64 ? 8 : 4;
+    const int vec = j.dtype == KG_F64 ? TypeOf<KG_F64>::VEC : TypeOf<KG_F32>::VEC;
+    const int es = j.dtype == KG_F64 ? 8 : 4;
     const size_t sy = (size_t)((j.nx + vec - 1) / vec * vec);
     return Arena::up(sy * j.ny * j.ns * es) * nlev_of(j.kind);
 }

@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 #include "../../include/kg_stencil.h"
 
@@ -65,6 +67,7 @@ int kg_preload_kernel(const void* func);
 int kg_check_ctx(kg_ctx* ctx);
 int kg_check_dims3(int nx, int ny, int ns);
 int kg_check_dims2(int nx, int ny);
+int kg_check_plane_bytes(long long points, int elem_bytes);
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; each XCD
 // has its own L2.  kg_xcd_swizzle maps the hardware workgroup id to a logical
@@ -79,6 +82,19 @@ __device__ __forceinline__ unsigned kg_xcd_swizzle(unsigned b, unsigned n)
 }
 
 #define KG_NUM_XCD 8
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}):
+// a loop unrolled at compile time with the index usable as a constant.
+template <class Fn, int... I>
+__device__ __forceinline__ void kg_static_for_impl(Fn&& f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class Fn>
+__device__ __forceinline__ void kg_static_for(Fn&& f)
+{
+    kg_static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // Streaming store for stencil outputs.  Output grids are written once per step
 // and not read again by the same kernel; marking the store non-temporal keeps

@@ -1,9 +1,11 @@
-// kg_ops3d.h -- stencil bodies on the plane engine (kg_plane.h) that exist in
-// several storage types: the FP64 drop-in kernels, the FP32/BF16 precision
-// variants and the batched/served grids all run these same bodies.  Each op
-// computes the VEC points of one lane at plane k (see kgp::View) in the
-// reference's source order, in the arithmetic type C::F, and stores them in
-// the storage type C::S.
+// kg_ops3d.h -- stencil bodies that exist in several storage types: the FP64
+// drop-in kernels, the FP32/BF16 precision variants and the batched/served
+// grids all run these same bodies.  Each op computes the VEC points of one
+// lane at plane k in the reference's source order, in the arithmetic type
+// C::F, and stores them in the storage type C::S.  Operands come through a
+// view: kgp::View (the plane engine: LDS and registers) or kgops::FlatView
+// (global memory, for small grids in the batch); both give the same values,
+// so the results are the same bit for bit.
 #pragma once
 
 #include "kg_plane.h"
@@ -19,7 +21,8 @@ struct LaplacianOp {
     const S* side[1];
     S* dst;
     F alpha, beta;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         F r[C::VEC];
@@ -43,7 +46,8 @@ struct Wave13ptOp {
     const S* side[1];
     S* dst;
     F m0, m1, m2;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         F r[C::VEC];
@@ -73,7 +77,8 @@ struct Gaussblur3Op {
     const S* side[1];
     S* dst;
     F f, s0, s1, s2, s4, s5, s8;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         F r[C::VEC];
@@ -105,7 +110,8 @@ struct Jacobi2Op {
     const S* side[1];
     S* dst;
     F c0, c1, c2;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         F r[C::VEC];
@@ -127,7 +133,8 @@ struct Life2Op {
     const S* src;
     const S* side[1];
     S* dst;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         F r[C::VEC];
@@ -156,6 +163,25 @@ template <> struct Bounds<Life2Op> { static constexpr int LO = 1, HI = 1, ZLO = 
 }  // namespace kgops
 
 namespace kgops {
+
+// Operands straight from global memory (VEC = 1): the point at offset o,
+// neighbours at o + dx + dy*sy + dz*sz.  For grids small enough that their
+// planes stay in the caches while a workgroup sweeps them.
+template <class C>
+struct FlatView {
+    using S = typename C::S;
+    using F = typename C::F;
+    using T = typename C::T;
+    const kgp::gmem<const S>* src;
+    const kgp::gmem<const S>* aux[C::NA];
+    long long o, sy, sz;
+    __device__ __forceinline__ F at(int, int dx, int dy, int dz) const
+    {
+        return kgp::ld_cvt<T>(src[o + dx + dy * sy + dz * sz]);
+    }
+    __device__ __forceinline__ F col(int, int dz) const { return kgp::ld_cvt<T>(src[o + dz * sz]); }
+    __device__ __forceinline__ F side(int a, int) const { return kgp::ld_cvt<T>(aux[a][o]); }
+};
 
 // Engine shape each op needs: halo width, LDS planes, register column and
 // side streams (kgp::Cfg parameters H, LL, LR, ZL, ZR, NAUX).

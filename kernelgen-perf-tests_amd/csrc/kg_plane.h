@@ -103,6 +103,44 @@ __device__ __forceinline__ void vload(const gmem<typename T::S>* p, typename T::
     for (int v = 0; v < VEC; v++) out[v] = ld_cvt<T>(s[v]);
 }
 
+// Buffer-resource access.  A buffer load whose offset lies at or past the
+// resource's num_records returns 0 without touching memory, so lanes outside
+// the grid, and planes outside the grid or past the tile, are handled by the
+// offset or by an empty resource instead of by a branch.  Keeping every load
+// out of conditional code matters: the compiler waits for an earlier load by
+// counting the memory operations issued after it on every path, and a load
+// that may be skipped does not count -- with all loads conditional it would
+// wait for the whole prefetch queue (vmcnt(0)) every plane.
+constexpr unsigned KG_OOB = 0x80000000u;          // offset past any resource used here
+constexpr int KG_RSRC_WORD3 = 0x00020000;         // gfx9 raw buffer: 32-bit data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, KG_RSRC_WORD3);
+}
+
+template <int BYTES>
+__device__ __forceinline__ typename Raw<BYTES>::t buf_load(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    if constexpr (BYTES == 16) return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    else if constexpr (BYTES == 8) return __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    else if constexpr (BYTES == 4) return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+    else return __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+}
+
+// VEC consecutive storage values at byte offset off of resource r.
+template <class T, int VEC>
+__device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                      typename T::F (&out)[VEC])
+{
+    using S = typename T::S;
+    const auto raw = buf_load<VEC * (int)sizeof(S)>(r, off);
+    S s[VEC];
+    __builtin_memcpy(s, &raw, sizeof raw);
+#pragma unroll
+    for (int v = 0; v < VEC; v++) out[v] = ld_cvt<T>(s[v]);
+}
+
 // ---------------------------------------------------------------- config
 template <int VEC_, int W_, int H_, int LL_, int LR_, int ZL_, int ZR_, int D_, int NAUX_, int KC_,
           class T_ = F64, int LPR_ = 64>
@@ -128,7 +166,8 @@ struct Cfg {
     static constexpr int NLS = LL + LR + 2;          // LDS plane slots
     static constexpr int ZLO = ZL;                   // ring starts at k-ZL
     static constexpr int ZHI = (ZR > LR ? ZR : LR);  // ring reaches k+ZHI before prefetch
-    static constexpr int RS = ZLO + ZHI + D;         // ring slots: planes k-ZL .. k+ZHI+D-1
+    static constexpr int RS = ZLO + ZHI + D;         // ring: planes k-ZL .. k+ZHI+D-1
+    static constexpr int NP = RS + 1;                // physical slots (one being filled)
     static constexpr int TY = ROWS - 2 * H;          // computed rows per tile
     static constexpr int NA = NAUX > 0 ? NAUX : 1;
     static constexpr size_t LDS_BYTES = sizeof(F) * NLS * ROWS * LW;
@@ -173,13 +212,16 @@ static inline Grid make_grid(int nx, int ny, int ns, int lo, int hi)
     return make_grid<C>(nx, ny, ns, lo, hi, lo, hi, C::KC);
 }
 
-// Per-lane view handed to a stencil's compute function.
-template <class C>
+// Per-lane view handed to a stencil's compute function at ring phase P: the
+// register ring's logical slot r (plane k - ZL + r) is physical slot
+// (P + r) % NP, and the side-stream ring's logical slot d (plane k + d) is
+// physical slot (P + d) % NP.
+template <class C, int P = 0>
 struct View {
     using F = typename C::F;
     const F (*lds)[C::ROWS][C::LW];            // [NLS][ROWS][LW]
-    const Slot<F, C::VEC>* ring;               // ring[ZLO + dz] = own row at plane k+dz
-    const ASlot<F, C::VEC, C::NA>* aux;        // aux[0] = side streams at plane k
+    const Slot<F, C::VEC>* ring;               // [NP]
+    const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
     int k, row, cl;   // plane, tile row and lane-in-row of this lane
     // main field at (x + v + dx, j + dy, k + dz) from LDS, |dx|,|dy| <= H,
     // -LL <= dz <= LR
@@ -189,9 +231,11 @@ struct View {
         return lds[s][row + dy][C::H + cl * C::VEC + v + dx];
     }
     // main field at (x + v, j, k + dz) from registers, -ZL <= dz <= ZHI
-    __device__ __forceinline__ F col_(int v, int dz) const { return ring[C::ZLO + dz].v[v]; }
-    __device__ __forceinline__ F col(int v, int dz) const { return col_(v, dz); }
-    __device__ __forceinline__ F side(int a, int v) const { return aux[0].a[a][v]; }
+    __device__ __forceinline__ F col(int v, int dz) const
+    {
+        return ring[(P + C::ZLO + dz) % C::NP].v[v];
+    }
+    __device__ __forceinline__ F side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
 };
 
 // One tile (bx, by, bz) of a grid: xy-interior [LO, n-HI), z-interior
@@ -230,32 +274,34 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
     bool p_ok[C::VEC];
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) p_ok[v] = row_ok && x + v < nx;
-    const bool vec_ok = C::VEC > 1 && p_ok[C::VEC - 1];
-    const gmem<const S>* src = as_global(op.src) + (long long)j * sy;
-
-    auto load = [&](int k, Slot<F, C::VEC>& s) {
-        if (k < 0 || k >= ns) return;
-        const gmem<const S>* pp = src + (long long)k * sz;
-        if (C::VEC > 1 && vec_ok) {
-            vload<T, C::VEC>(pp + x, s.v);
-        } else {
-#pragma unroll
-            for (int v = 0; v < C::VEC; v++) s.v[v] = p_ok[v] ? ld_cvt<T>(pp[x + v]) : F(0);
-        }
-        s.h = h_ok ? ld_cvt<T>(pp[hx]) : F(0);
+    // Per-lane byte offsets within a plane, fixed for the whole tile (KG_OOB:
+    // nothing to load).  Planes are addressed through one resource each:
+    // base + k*sz, sz*sizeof(S) bytes (at most 2^31; the launchers check).
+    const unsigned plane_bytes = (unsigned)(sz * (long long)sizeof(S));
+    const unsigned off_v = p_ok[0] ? (unsigned)(((long long)j * sy + x) * (long long)sizeof(S)) : KG_OOB;
+    const unsigned off_h = h_ok ? (unsigned)(((long long)j * sy + hx) * (long long)sizeof(S)) : KG_OOB;
+    const unsigned off_a = comp_row ? off_v : KG_OOB;
+    // Resource for plane k of array base; empty unless want and 0 <= k < ns.
+    auto plane_rsrc = [&](const S* base, int k, bool want) {
+        const bool ok = want && k >= 0 && k < ns;
+        return make_rsrc(base + (long long)(ok ? k : 0) * sz, ok ? plane_bytes : 0u);
     };
-    auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s) {
-        if (C::NAUX == 0 || !comp_row || k >= ns) return;
+
+    // One access per lane and slot (VEC values, then the halo value); values
+    // a lane does not load (zeros) are never read for an interior point.  With
+    // VEC > 1 every lane with x < nx reads VEC values: callers guarantee that
+    // rows hold a multiple of VEC elements (nx % VEC == 0 in the dense
+    // layout, padded rows otherwise).
+    auto load = [&](int k, Slot<F, C::VEC>& s, bool want = true) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(op.src, k, want);
+        bload<T, C::VEC>(r, off_v, s.v);
+        F h[1];
+        bload<T, 1>(r, off_h, h);
+        s.h = h[0];
+    };
+    auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s, bool want = true) {
 #pragma unroll
-        for (int a = 0; a < C::NAUX; a++) {
-            const gmem<const S>* pp = as_global(op.side[a]) + (long long)j * sy + (long long)k * sz;
-            if (C::VEC > 1 && vec_ok) {
-                vload<T, C::VEC>(pp + x, s.a[a]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < C::VEC; v++) s.a[a][v] = p_ok[v] ? ld_cvt<T>(pp[x + v]) : F(0);
-            }
-        }
+        for (int a = 0; a < C::NAUX; a++) bload<T, C::VEC>(plane_rsrc(op.side[a], k, want), off_a, s.a[a]);
     };
     auto stage = [&](int k, const Slot<F, C::VEC>& s) {
         F(*L)[C::LW] = lds[k % C::NLS];
@@ -265,10 +311,15 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
         else if (c >= C::LPR - C::H) L[row][C::H + C::TXW + (c - (C::LPR - C::H))] = s.h;
     };
 
-    Slot<F, C::VEC> ring[C::RS];
-    ASlot<F, C::VEC, C::NA> aux[C::D];
+    // Register rings of NP = RS + 1 physical slots.  The plane loop is
+    // unrolled NP times and each copy uses compile-time slot indices (phase
+    // P: logical slot r is physical (P + r) % NP), so a prefetch stays in
+    // flight in its own registers until it is used D planes later: no register
+    // copies, hence no wait for loads issued in the same iteration.
+    Slot<F, C::VEC> ring[C::NP];
+    ASlot<F, C::VEC, C::NA> aux[C::NP];
 #pragma unroll
-    for (int r = 0; r < C::RS; r++) {
+    for (int r = 0; r < C::NP; r++) {
 #pragma unroll
         for (int v = 0; v < C::VEC; v++) ring[r].v[v] = F(0);
         ring[r].h = F(0);
@@ -295,28 +346,27 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
 
-    for (int k = k0; k < k1; k++) {
-        stage(k + C::LR, ring[C::ZLO + C::LR]);
-        Slot<F, C::VEC> nq;
-        nq.h = F(0);
-#pragma unroll
-        for (int v = 0; v < C::VEC; v++) nq.v[v] = F(0);
-        ASlot<F, C::VEC, C::NA> na;
-        const int kp = k + C::ZHI + C::D;    // plane entering the ring
-        if (kp <= k1 - 1 + C::ZHI) load(kp, nq);
-        if (k + C::D < k1) load_aux(k + C::D, na);
+    // Plane k at phase P.  The planes the ring and LDS need are all < ns
+    // (the stencil reach ZHI <= ZB1), so slots whose load is skipped are
+    // never read.
+    auto plane = [&](auto phase, int k) {
+        constexpr int P = decltype(phase)::value;
+        stage(k + C::LR, ring[(P + C::ZLO + C::LR) % C::NP]);
+        const int kp = k + C::ZHI + C::D;   // plane entering the ring
+        load(kp, ring[(P + C::RS) % C::NP], kp <= k1 - 1 + C::ZHI);
+        load_aux(k + C::D, aux[(P + C::D) % C::NP], k + C::D < k1);
         __syncthreads();
         if (comp_row) {
-            View<C> vw{(const F(*)[C::ROWS][C::LW])lds, ring, aux, k, row, c};
+            View<C, P> vw{(const F(*)[C::ROWS][C::LW])lds, ring, aux, k, row, c};
             op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
-#pragma unroll
-        for (int r = 0; r < C::RS - 1; r++) ring[r] = ring[r + 1];
-        ring[C::RS - 1] = nq;
-#pragma unroll
-        for (int d = 0; d < C::D - 1; d++) aux[d] = aux[d + 1];
-        aux[C::D - 1] = na;
-    }
+    };
+    int k = k0;
+    for (; k + C::NP <= k1; k += C::NP)
+        kg_static_for<C::NP>([&](auto p) { plane(p, k + decltype(p)::value); });
+    kg_static_for<C::NP - 1>([&](auto p) {
+        if (k + decltype(p)::value < k1) plane(p, k + decltype(p)::value);
+    });
 }
 
 // Single-grid launch in the dense layout: one workgroup per tile, tiles

@@ -84,6 +84,16 @@ int kg_check_dims3(int nx, int ny, int ns)
     return KG_OK;
 }
 
+// The plane engine addresses each xy-plane through one buffer resource with
+// 32-bit byte offsets: a plane may hold at most 2^31 bytes.
+int kg_check_plane_bytes(long long points, int elem_bytes)
+{
+    if (points * elem_bytes > (1LL << 31))
+        return kg_set_error(KG_ERR_INVALID, "xy-plane of %lld points x %d bytes exceeds 2^31 bytes",
+                            points, elem_bytes);
+    return KG_OK;
+}
+
 int kg_check_dims2(int nx, int ny)
 {
     if (nx < 0 || ny < 0)

@@ -100,7 +100,8 @@ struct GradientOp {
     const double* side[1];
     double *ux, *uy, *uz;
     double alpha, beta, gamma;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         double rx[C::VEC], ry[C::VEC], rz[C::VEC];
@@ -123,7 +124,8 @@ struct LapgsrbOp {
     const double* side[1];
     double* dst;
     double c0, c1, c2, c3;
-    __device__ __forceinline__ void run(const kgp::View<C>& p, long long ob, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long ob, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         double r[C::VEC];
@@ -162,6 +164,7 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, F
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
     if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+    if (int rc = kg_check_plane_bytes((long long)nx * ny, 8)) return rc;
     auto go = [&](auto vtag) -> int {
         constexpr int V = decltype(vtag)::value;
         using C = CfgT<V>;

@@ -13,6 +13,7 @@
 // partials), so a result does not depend on timing and repeats bit for bit.
 #include "kg_ops3d.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace {
@@ -29,6 +30,7 @@ static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, F
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
     if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+    if (int rc = kg_check_plane_bytes((long long)nx * ny, (int)sizeof(typename T::S))) return rc;
     constexpr int LO = Bounds<OpT>::LO, HI = Bounds<OpT>::HI;
     constexpr int ZLO = Bounds<OpT>::ZLO, ZHI = Bounds<OpT>::ZHI;
     auto go = [&](auto vtag) -> int {
@@ -154,7 +156,47 @@ static int stats_dispatch_ref(hipStream_t st, int rdt, size_t n, const void* x, 
     return KG_OK;
 }
 
+// ------------------------------------------------------------ conversion
+template <int DS, int DD>
+__global__ __launch_bounds__(256)
+void convert_k(size_t n, const void* __restrict__ src, void* __restrict__ dst)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+        const double v = ld_any<DS>(src, i);
+        if constexpr (DD == KG_F64) ((double*)dst)[i] = v;
+        else if constexpr (DD == KG_F32) ((float*)dst)[i] = (float)v;
+        // double -> float -> bfloat16: exact except where the float rounding
+        // decides a bfloat16 tie (not for values that came from bfloat16/float)
+        else ((uint16_t*)dst)[i] = kgp::bf16_out((float)v);
+    }
+}
+
+template <int DS>
+static int convert_to(hipStream_t st, int dd, size_t n, const void* src, void* dst)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 32);
+    if (n == 0) return KG_OK;
+    if (dd == KG_F64) hipLaunchKernelGGL((convert_k<DS, KG_F64>), dim3(blocks), dim3(256), 0, st, n, src, dst);
+    else if (dd == KG_F32) hipLaunchKernelGGL((convert_k<DS, KG_F32>), dim3(blocks), dim3(256), 0, st, n, src, dst);
+    else if (dd == KG_BF16) hipLaunchKernelGGL((convert_k<DS, KG_BF16>), dim3(blocks), dim3(256), 0, st, n, src, dst);
+    else return kg_set_error(KG_ERR_INVALID, "kg_convert: unknown dtype %d", dd);
+    KG_HIP(hipGetLastError());
+    return KG_OK;
+}
+
 }  // namespace
+
+extern "C" int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, const void* src,
+                          size_t n)
+{
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (n && (!dst || !src)) return kg_set_error(KG_ERR_INVALID, "kg_convert: null buffer");
+    if (src_dtype == KG_F64) return convert_to<KG_F64>(ctx->stream, dst_dtype, n, src, dst);
+    if (src_dtype == KG_F32) return convert_to<KG_F32>(ctx->stream, dst_dtype, n, src, dst);
+    if (src_dtype == KG_BF16) return convert_to<KG_BF16>(ctx->stream, dst_dtype, n, src, dst);
+    return kg_set_error(KG_ERR_INVALID, "kg_convert: unknown dtype %d", src_dtype);
+}
 
 // ======================================================= precision variants
 

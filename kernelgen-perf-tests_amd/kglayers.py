@@ -1,0 +1,212 @@
+"""Measurement drivers for the layers beside the FP64 drop-in kernels:
+batching, serving and precision, each at the configuration it is specified
+for.  Every function takes the kgstencil module (``kg``) and an open
+``kg.Context``; all computation runs in libkgstencil.so on the device.
+
+* ``run_batch``: 4096 independent 3D grids, every edge log-uniform in
+  [16, 256] from a seed, kinds drawn uniformly from laplacian / gaussblur
+  (per xy-plane) / wave13pt, 100 steps under one captured graph, tile shapes
+  autotuned per size class; the same grids stepped by one plain launch per
+  grid and step for comparison.
+* ``run_serve``: Poisson arrivals of jobs (kind uniform over laplacian,
+  wave13pt, gaussblur, jacobi-to-1e-8 and gameoflife; volume log-uniform from
+  32^3 to 512x256x256; steps log-uniform in [10, 1000]; FP64 or FP32 with
+  p = 0.5) admitted against a 250 GB HBM budget, at several offered loads,
+  continuous batching against one-job-at-a-time plain launches.
+* ``run_precision``: laplacian and wave13pt at 512^3 for 1000 steps in FP64,
+  FP32 and BF16 storage, with errors of the FP32/BF16 fields against the FP64
+  run (bit-identical to the oracle) and FP64 norms/energy.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+# Coefficients chosen stable for long runs (see run_precision):
+#  laplacian alpha = 1/4, beta = 1/8: amplification alpha + beta*sum(2 cos) in [-1/2, 1];
+#  wave13pt C = (c dt/h)^2 = 3/32 of the 4th-order leapfrog (stable for C <= 1/4):
+#   m0 = 2 - 7.5 C, m1 = 4C/3, m2 = -C/12, all exact in FP32 and BF16;
+#  gaussblur sigma = 1: s(r^2) = exp(-r^2/2);
+#  jacobi: the 9-point average c0 = 0, c1 = 1/5, c2 = 1/20.
+LAP = (0.25, 0.125)
+WAVE = (1.296875, 0.125, -0.0078125)
+GAUSS = tuple(math.exp(-r2 / 2.0) for r2 in (0, 1, 2, 4, 5, 8))
+JAC = (0.0, 0.2, 0.05)
+COEFS = {"laplacian": LAP, "wave13pt": WAVE, "gaussblur3": GAUSS, "jacobi2": JAC, "life2": ()}
+HBM_PEAK_GBS = 8000.0
+
+
+def _log_uniform(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
+
+
+# ----------------------------------------------------------------- batching
+def batch_grids(seed: int = 1, ngrids: int = 4096, lo: int = 16, hi: int = 256):
+    rng = np.random.default_rng(seed)
+    edges = np.minimum(np.floor(_log_uniform(rng, lo, hi + 1, (ngrids, 3))), hi).astype(int)
+    kinds = rng.choice(["laplacian", "gaussblur3", "wave13pt"], ngrids)
+    return [(str(k), tuple(int(v) for v in e)) for k, e in zip(kinds, edges)]
+
+
+def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int = 4096,
+              tune_reps: int = 2, seq_steps: int = 3) -> dict:
+    grids = batch_grids(seed, ngrids)
+    b = kg.Batch(ctx)
+    try:
+        for kind, dims in grids:
+            b.add(kind, dtype, dims, COEFS[kind])
+        b.fill(seed)
+        t0 = time.perf_counter()
+        ntuned = b.tune(tune_reps)
+        t_tune = time.perf_counter() - t0
+        info = b.info()
+        reserved, in_use = b.memory()
+        b.step(1, graph=False)                 # warm
+        b.step(steps, graph=True)              # capture + first replay
+        ctx.sync()
+        t0 = time.perf_counter()
+        b.step(steps, graph=True)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        b.step_sequential(seq_steps)
+        ctx.sync()
+        t_seq = time.perf_counter() - t0
+    finally:
+        b.close()
+    pts = info["points_per_step"]
+    return {
+        "grids": ngrids, "steps": steps, "dtype": "f64" if dtype == kg.F64 else "f32",
+        "points_per_step": pts, "hbm_bytes_resident": in_use, "arena_reserved": reserved,
+        "work_items": info["work_items"], "workgroups": info["workgroups"],
+        "classes_tuned": ntuned, "tune_s": round(t_tune, 3),
+        "ms_per_step": 1e3 * t / steps,
+        "gpts_s": pts * steps / t / 1e9,
+        "gbs": info["bytes_per_step"] * steps / t / 1e9,
+        "frac_hbm": info["bytes_per_step"] * steps / t / 1e9 / HBM_PEAK_GBS,
+        "sequential_ms_per_step": 1e3 * t_seq / seq_steps,
+        "sequential_gpts_s": pts * seq_steps / t_seq / 1e9,
+    }
+
+
+# ------------------------------------------------------------------ serving
+def serve_jobs(n: int, rate: float, seed: int = 7, vmin: float = 32 ** 3,
+               vmax: float = 512 * 256 * 256) -> list[dict]:
+    """n jobs with exponential inter-arrival times of mean 1/rate seconds."""
+    rng = np.random.default_rng(seed)
+    kinds = ["laplacian", "wave13pt", "gaussblur3", "jacobi2", "life2"]
+    t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+    jobs = []
+    for i in range(n):
+        kind = kinds[int(rng.integers(0, 5))]
+        v = float(_log_uniform(rng, vmin, vmax))
+        s = (v / (512 * 256 * 256)) ** (1.0 / 3.0)
+        nx, ny, ns = max(3, round(512 * s)), max(3, round(256 * s)), max(3, round(256 * s))
+        if kind in ("jacobi2", "life2"):       # 2D problem nx x (ny*ns), as the 2D tests run
+            ny, ns = ny * ns, 1
+        steps = int(round(float(_log_uniform(rng, 10, 1000))))
+        dtype = 0 if rng.random() < 0.5 else 1
+        jobs.append({"kind": kind, "dtype": dtype, "dims": (nx, ny, ns), "steps": steps,
+                     "coef": COEFS[kind], "tol": 1e-8 if kind == "jacobi2" else 0.0,
+                     "arrival": float(t[i]), "seed": 100 + i})
+    return jobs
+
+
+def run_serve(kg, ctx, njobs: int = 120, loads=(0.5, 0.8, 1.0, 1.5), seed: int = 7,
+              budget: int = 250 * 10 ** 9, max_active: int = 256) -> dict:
+    # capacity: the same jobs, all present at t = 0, continuous batching
+    base = serve_jobs(njobs, 0.0, seed)
+    _, st0, _ = kg.serve(ctx, base, budget, max_active)
+    cap = njobs / st0["wall"]
+    out = {"jobs": njobs, "budget_bytes": budget, "capacity_jobs_s": cap,
+           "capacity_gpts_s": st0["points"] / st0["wall"] / 1e9, "loads": []}
+    for load in loads:
+        jobs = serve_jobs(njobs, load * cap, seed)
+        row = {"offered_load": load, "rate_jobs_s": load * cap}
+        for name, mode in (("continuous", kg.SERVE_CONTINUOUS), ("sequential", kg.SERVE_SEQUENTIAL)):
+            res, st, _ = kg.serve(ctx, jobs, budget, max_active, mode)
+            ok = [r for r in res if r["status"] == 0]
+            row[name] = {"p50_s": st["p50"], "p99_s": st["p99"], "mean_s": st["mean_latency"],
+                         "gpts_s": st["points"] / st["wall"] / 1e9, "wall_s": st["wall"],
+                         "peak_active": st["peak_active"], "peak_bytes": st["peak_bytes"],
+                         "completed": len(ok)}
+        out["loads"].append(row)
+    return out
+
+
+# ---------------------------------------------------------------- precision
+def run_precision(kg, ctx, n: int = 512, steps: int = 1000, check_every: int = 250,
+                  time_steps: int = 50, seed: int = 3) -> dict:
+    """FP64 / FP32 / BF16 runs in lockstep from the same FP64 initial field;
+    errors of the FP32 and BF16 fields against the FP64 one (max abs and
+    relative L2), FP64 sums of squares (and, for wave13pt, the kinetic term
+    sum (w_new - w_old)^2) at checkpoints, and time per step per dtype."""
+    N = n * n * n
+    out = {"grid": [n, n, n], "steps": steps}
+    dts = [("f64", kg.F64), ("f32", kg.F32), ("bf16", kg.BF16)]
+    for kind, nlev in (("laplacian", 2), ("wave13pt", 3)):
+        co = COEFS[kind]
+        lv = {}
+        for name, code in dts:
+            lv[name] = [ctx.empty((N,), {"f64": np.float64, "f32": np.float32,
+                                         "bf16": np.uint16}[name]) for _ in range(nlev)]
+        for l in range(nlev):
+            kg_fill = ctx.lib.kg_fill_uniform
+            _c = kg._check
+            _c(ctx.lib, kg_fill(ctx.ptr, kg.F64, N, lv["f64"][l].ptr, seed * 10 + l), "fill")
+            for name, code in dts[1:]:
+                _c(ctx.lib, ctx.lib.kg_convert(ctx.ptr, code, lv[name][l].ptr, kg.F64,
+                                               lv["f64"][l].ptr, N), "convert")
+        fn = {"f64": kind, "f32": kind + "_f32", "bf16": kind + "_bf16"}
+
+        def step(name, bufs):
+            if kind == "wave13pt":
+                ctx.call(fn[name], n, n, n, *co, bufs[0], bufs[1], bufs[2])
+                return [bufs[1], bufs[2], bufs[0]]
+            ctx.call(fn[name], n, n, n, *co, bufs[0], bufs[1])
+            return [bufs[1], bufs[0]]
+
+        def newest(bufs):
+            return bufs[1] if kind == "wave13pt" else bufs[0]
+
+        def older(bufs):
+            return bufs[0] if kind == "wave13pt" else bufs[1]
+
+        checks = []
+        for s in range(1, steps + 1):
+            for name, _ in dts:
+                lv[name] = step(name, lv[name])
+            if s % check_every == 0 or s == steps:
+                ref = newest(lv["f64"])
+                st64 = ctx.field_stats(ref, kg.F64)
+                kin64 = ctx.field_stats(ref, kg.F64, older(lv["f64"]), kg.F64)["err_sumsq"]
+                row = {"step": s, "f64_sumsq": st64["sumsq"], "f64_kinetic": kin64}
+                for name, code in dts[1:]:
+                    e = ctx.field_stats(newest(lv[name]), code, ref, kg.F64)
+                    kin = ctx.field_stats(newest(lv[name]), code, older(lv[name]), code)["err_sumsq"]
+                    row[name] = {"max_abs_err": e["err_max"],
+                                 "rel_l2_err": math.sqrt(e["err_sumsq"] / max(st64["sumsq"], 1e-300)),
+                                 "sumsq_rel_drift": e["sumsq"] / st64["sumsq"] - 1.0,
+                                 "kinetic_rel_drift": (kin / kin64 - 1.0) if kin64 > 0 else 0.0}
+                checks.append(row)
+        timing = {}
+        for name, _ in dts:
+            bufs = lv[name]
+            for _ in range(3):
+                bufs = step(name, bufs)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(time_steps):
+                bufs = step(name, bufs)
+            ctx.sync()
+            t = (time.perf_counter() - t0) / time_steps
+            bpp = {"f64": 8, "f32": 4, "bf16": 2}[name] * (3 if kind == "wave13pt" else 2)
+            timing[name] = {"ms_per_step": 1e3 * t, "gpts_s": N / t / 1e9,
+                            "gbs": N * bpp / t / 1e9, "frac_hbm": N * bpp / t / 1e9 / HBM_PEAK_GBS}
+        for name, _ in dts:
+            for buf in lv[name]:
+                buf.free()
+        out[kind] = {"coef": co, "checks": checks, "timing": timing}
+    return out

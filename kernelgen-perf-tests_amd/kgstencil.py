@@ -47,6 +47,7 @@ SIGNATURES = {
     "kg_gaussblur3_f32": [_i, _i, _i, _f, _f, _f, _f, _f, _f, _dp, _dp],
     "kg_field_stats": [_i, C.c_size_t, _dp, _i, _dp, C.POINTER(_d)],
     "kg_fill_uniform": [_i, C.c_size_t, _dp, C.c_ulonglong],
+    "kg_convert": [_i, _dp, _i, _dp, C.c_size_t],
     "kg_matmul": [_i, _i, _i, _dp, _dp, _dp],
     "kg_stream_copy": [_dp, _dp, C.c_size_t],
 }

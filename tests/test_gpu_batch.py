@@ -153,16 +153,16 @@ def test_batch_shapes_and_tuning_do_not_change_results(gpu_ctx, batch):
     specs = _mixed_specs(rng, 9, 5, 90)
     ids, init = _fill(batch, specs, rng)
     batch.step(1, graph=False)
-    for lpr in (16, 32, 64):
+    for lpr in (16, 32, 64, 0):        # 0: the flat (cache-served) body
         for kc in (1, 7, 64):
             batch.set_shape(ids[0], lpr, kc)
             batch.set_shape(ids[4], lpr, kc)
             assert batch.shape(ids[0]) == (lpr, kc)
             batch.step(1, graph=True)
-    _check_all(gpu_ctx, batch, specs, ids, init, 10, "forced shapes")
+    _check_all(gpu_ctx, batch, specs, ids, init, 13, "forced shapes")
     assert batch.tune(reps=1) >= 1
     batch.step(2, graph=True)
-    _check_all(gpu_ctx, batch, specs, ids, init, 12, "after tuning")
+    _check_all(gpu_ctx, batch, specs, ids, init, 15, "after tuning")
 
 
 def test_batch_grids_join_and_leave(gpu_ctx, batch):
