@@ -10,10 +10,13 @@
 //   * every global load is issued D rows ahead of use (a register ring per
 //     field, plus a ring for NAUX side streams read only at the point), so the
 //     loop's vector-memory queue holds only this prefetch stream and the
-//     output stores.
+//     output stores.  As in kg_plane.h the rings rotate by renaming (the row
+//     loop is unrolled RS+1 times) and loads go through buffer resources with
+//     out-of-range offsets for idle lanes, so no load sits in conditional code
+//     and the compiler waits only for the row it is about to use.
 #pragma once
 
-#include "kg_internal.h"
+#include "kg_plane.h"
 
 namespace kgr {
 
@@ -38,6 +41,7 @@ struct Cfg {
     static constexpr int NLS = LL + LR + 2;
     static constexpr int ZHI = (ZR > LR ? ZR : LR);
     static constexpr int RS = ZL + ZHI + D;          // ring rows j-ZL .. j+ZHI+D-1
+    static constexpr int NP = RS + 1;                // physical slots (one being filled)
     static constexpr int NA = NAUX > 0 ? NAUX : 1;
 };
 
@@ -70,11 +74,14 @@ static inline Grid make_grid(int nx, int ny, int lo, int hi)
     return g;
 }
 
-template <class C>
+// View at ring phase P: logical ring row r (row j - ZL + r) of field f is
+// physical slot (P + r) % NP; logical side-stream row d (row j + d) is
+// physical slot (P + d) % NP.
+template <class C, int P = 0>
 struct View {
     const double* lds;                        // [NF][NLS][LW]
-    const Slot<C::VEC> (*ring)[C::RS];        // [NF][RS]
-    const ASlot<C::VEC, C::NA>* aux;          // aux[0] = side streams at row j
+    const Slot<C::VEC> (*ring)[C::NP];        // [NF][NP]
+    const ASlot<C::VEC, C::NA>* aux;          // [NP]
     int j, col;                               // col: LDS column of the lane's first point
     // field f at (x + v + dx, j + dy), |dx| <= H, -LL <= dy <= LR
     __device__ __forceinline__ double at(int f, int v, int dx, int dy) const
@@ -82,8 +89,11 @@ struct View {
         return lds[(f * C::NLS + (j + dy) % C::NLS) * C::LW + col + v + dx];
     }
     // field f at (x + v, j + dy), -ZL <= dy <= ZHI
-    __device__ __forceinline__ double col_(int f, int v, int dy) const { return ring[f][C::ZL + dy].v[v]; }
-    __device__ __forceinline__ double side(int a, int v) const { return aux[0].a[a][v]; }
+    __device__ __forceinline__ double col_(int f, int v, int dy) const
+    {
+        return ring[f][(P + C::ZL + dy) % C::NP].v[v];
+    }
+    __device__ __forceinline__ double side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
 };
 
 // Op supplies src[NF], side[NAUX] and run(view, o, x, xmask) which computes
@@ -107,31 +117,27 @@ void row_kernel(int nx, int ny, int ntx, Op op)
     else if (w == C::W - 1 && lane >= 64 - C::H) hx = x0 + C::TXW + (lane - (64 - C::H));
     const int hcol = hx < 0 ? -1 : hx - x0 + C::H;
     const bool h_ok = hx >= 0 && hx < nx;
-    bool p_ok[C::VEC];
-#pragma unroll
-    for (int v = 0; v < C::VEC; v++) p_ok[v] = x + v < nx;
-    const bool vec_ok = C::VEC == 2 && p_ok[0] && p_ok[C::VEC - 1];
-
-    auto load_row = [&](const double* base, int row, double (&v)[C::VEC]) {
-        const double* pp = base + (long long)row * nx;
-        if (C::VEC == 2 && vec_ok) {
-            const d2 t = *(const d2*)(pp + x);
-            v[0] = t.x;
-            v[C::VEC - 1] = t.y;
-        } else {
-#pragma unroll
-            for (int q = 0; q < C::VEC; q++) v[q] = p_ok[q] ? pp[x + q] : 0.0;
-        }
+    // Per-lane byte offsets within a row (kgp::KG_OOB: nothing to load).  With
+    // VEC = 2 callers guarantee an even nx, so a lane with x < nx may read two.
+    const unsigned off_v = x < nx ? (unsigned)x * 8u : kgp::KG_OOB;
+    const unsigned off_h = h_ok ? (unsigned)hx * 8u : kgp::KG_OOB;
+    const unsigned row_bytes = (unsigned)nx * 8u;
+    // Resource for row `row` of array base; empty unless want and 0 <= row < ny.
+    auto row_rsrc = [&](const double* base, int row, bool want) {
+        const bool ok = want && row >= 0 && row < ny;
+        return kgp::make_rsrc(base + (long long)(ok ? row : 0) * nx, ok ? row_bytes : 0u);
     };
-    auto load = [&](int f, int row, Slot<C::VEC>& s) {
-        if (row < 0 || row >= ny) return;
-        load_row(op.src[f], row, s.v);
-        s.h = h_ok ? op.src[f][(long long)row * nx + hx] : 0.0;
+    auto load = [&](int f, int row, Slot<C::VEC>& s, bool want = true) {
+        const __amdgpu_buffer_rsrc_t r = row_rsrc(op.src[f], row, want);
+        kgp::bload<kgp::F64, C::VEC>(r, off_v, s.v);
+        double h[1];
+        kgp::bload<kgp::F64, 1>(r, off_h, h);
+        s.h = h[0];
     };
-    auto load_aux = [&](int row, ASlot<C::VEC, C::NA>& s) {
-        if (C::NAUX == 0 || row >= ny) return;
+    auto load_aux = [&](int row, ASlot<C::VEC, C::NA>& s, bool want = true) {
 #pragma unroll
-        for (int a = 0; a < C::NAUX; a++) load_row(op.side[a], row, s.a[a]);
+        for (int a = 0; a < C::NAUX; a++)
+            kgp::bload<kgp::F64, C::VEC>(row_rsrc(op.side[a], row, want), off_v, s.a[a]);
     };
     auto stage = [&](int f, int row, const Slot<C::VEC>& s) {
         double* L = lds + (f * C::NLS + row % C::NLS) * C::LW;
@@ -140,12 +146,12 @@ void row_kernel(int nx, int ny, int ntx, Op op)
         if (hcol >= 0) L[hcol] = s.h;
     };
 
-    Slot<C::VEC> ring[C::NF][C::RS];
-    ASlot<C::VEC, C::NA> aux[C::D];
+    Slot<C::VEC> ring[C::NF][C::NP];
+    ASlot<C::VEC, C::NA> aux[C::NP];
 #pragma unroll
     for (int f = 0; f < C::NF; f++)
 #pragma unroll
-        for (int r = 0; r < C::RS; r++) {
+        for (int r = 0; r < C::NP; r++) {
 #pragma unroll
             for (int v = 0; v < C::VEC; v++) ring[f][r].v[v] = 0.0;
             ring[f][r].h = 0.0;
@@ -164,9 +170,6 @@ void row_kernel(int nx, int ny, int ntx, Op op)
                 stage(f, j0 + dy, ring[f][C::ZL + dy]);
             } else {
                 Slot<C::VEC> t;
-                t.h = 0.0;
-#pragma unroll
-                for (int v = 0; v < C::VEC; v++) t.v[v] = 0.0;
                 load(f, j0 + dy, t);
                 stage(f, j0 + dy, t);
             }
@@ -175,33 +178,25 @@ void row_kernel(int nx, int ny, int ntx, Op op)
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
 
-    for (int j = j0; j < j1; j++) {
+    // Row j at phase P (rows the ring and LDS need are all < ny).
+    auto row_step = [&](auto phase, int j) {
+        constexpr int P = decltype(phase)::value;
 #pragma unroll
-        for (int f = 0; f < C::NF; f++) stage(f, j + C::LR, ring[f][C::ZL + C::LR]);
-        Slot<C::VEC> nq[C::NF];
+        for (int f = 0; f < C::NF; f++) stage(f, j + C::LR, ring[f][(P + C::ZL + C::LR) % C::NP]);
         const int jp = j + C::ZHI + C::D;
 #pragma unroll
-        for (int f = 0; f < C::NF; f++) {
-            nq[f].h = 0.0;
-#pragma unroll
-            for (int v = 0; v < C::VEC; v++) nq[f].v[v] = 0.0;
-            if (jp <= j1 - 1 + C::ZHI) load(f, jp, nq[f]);
-        }
-        ASlot<C::VEC, C::NA> na;
-        if (j + C::D < j1) load_aux(j + C::D, na);
+        for (int f = 0; f < C::NF; f++) load(f, jp, ring[f][(P + C::RS) % C::NP], jp <= j1 - 1 + C::ZHI);
+        load_aux(j + C::D, aux[(P + C::D) % C::NP], j + C::D < j1);
         __syncthreads();
-        View<C> vw{lds, (const Slot<C::VEC>(*)[C::RS])ring, aux, j, col};
+        View<C, P> vw{lds, (const Slot<C::VEC>(*)[C::NP])ring, aux, j, col};
         op.run(vw, (long long)j * nx, x, xmask);
-#pragma unroll
-        for (int f = 0; f < C::NF; f++) {
-#pragma unroll
-            for (int r = 0; r < C::RS - 1; r++) ring[f][r] = ring[f][r + 1];
-            ring[f][C::RS - 1] = nq[f];
-        }
-#pragma unroll
-        for (int d = 0; d < C::D - 1; d++) aux[d] = aux[d + 1];
-        aux[C::D - 1] = na;
-    }
+    };
+    int j = j0;
+    for (; j + C::NP <= j1; j += C::NP)
+        kg_static_for<C::NP>([&](auto p) { row_step(p, j + decltype(p)::value); });
+    kg_static_for<C::NP - 1>([&](auto p) {
+        if (j + decltype(p)::value < j1) row_step(p, j + decltype(p)::value);
+    });
 }
 
 template <int VEC>

@@ -17,7 +17,8 @@ struct JacobiOp {
     const double* side[1];
     double* dst;
     double c0, c1, c2;
-    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long o, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         double r[C::VEC];
@@ -37,7 +38,8 @@ struct GaussblurOp {
     const double* side[1];
     double* dst;
     double f, s0, s1, s2, s4, s5, s8;
-    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long o, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         double r[C::VEC];
@@ -64,7 +66,8 @@ struct GameoflifeOp {
     const double* src[1];
     const double* side[1];
     double* dst;
-    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long o, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         double r[C::VEC];
@@ -97,7 +100,8 @@ struct WhisperingOp {
     const double* side[3];
     double *eo0, *eo1, *h1, *u_em1;
     double mu, epsilon;
-    __device__ __forceinline__ void run(const kgr::View<C>& p, long long o, int x,
+    template <class V>
+    __device__ __forceinline__ void run(const V& p, long long o, int x,
                                         const bool (&xm)[C::VEC]) const
     {
         double r0[C::VEC], r1[C::VEC], rh[C::VEC], ru[C::VEC];
@@ -131,11 +135,13 @@ struct WhisperingOp {
     }
 };
 
-// Row chunks from a sweep at 512x65536 (cold caches, 48-launch means): jacobi
-// YC=64 115 us (vs 118 at 32); whispering YC=8 612 us (vs 650 at 32).
+// Row shapes from sweeps at 512x65536 (cold caches, 40-launch means) with
+// the prefetch rings kept in flight: jacobi/gameoflife D=2/YC=64 113 us (D=4
+// 141, W=8 118); gaussblur D=2/YC=32 116 us (D=3/YC=64 140); whispering with
+// 16-byte lanes, W=4/D=2/YC=8 589 us (8-byte lanes 610-643).
 //                     VEC W      H  NF LL LR ZL ZR D  NAUX YC
-template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 3, 0, 64>;
-template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 3, 0, 32>;
+template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 2, 0, 64>;
+template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 2, 0, 32>;
 template <int V> using WhCfg = kgr::Cfg<V, 8 / V, 1, 5, 0, 0, 1, 1, 2, 3, 8>;
 
 // Launch Op on the row engine; 16-byte lanes (VEC=2) need an even nx.
@@ -223,7 +229,7 @@ extern "C" int kg_whispering(kg_ctx* ctx, int nx, int ny, double mu, double epsi
     KG_REQUIRE(e_in != e_out && h0 != h1 && u_em0 != u_em1,
                "kg_whispering: e, h and u_em need distinct in/out buffers");
     const long long n = (long long)nx * ny;
-    return launch_row<WhCfg, WhisperingOp, 1, 1, 1>(ctx, "whispering", nx, ny, [&](auto& op) {
+    return launch_row<WhCfg, WhisperingOp, 1, 1, 2>(ctx, "whispering", nx, ny, [&](auto& op) {
         op.src[0] = h0;
         op.src[1] = ca;
         op.src[2] = cb;

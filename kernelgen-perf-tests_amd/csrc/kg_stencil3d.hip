@@ -148,15 +148,15 @@ struct LapgsrbOp {
     }
 };
 
-// Tile shapes from a sweep at 512x256x256 (cold caches, 48-launch means):
-// laplacian W=8/D=2/KC=16 102 us (vs 108 at D=3/KC=32); wave13pt W=16 153 us
-// (vs 222 at W=12/D=3); lapgsrb with 8-byte lanes and W=16 161 us (vs 187 with
-// 16-byte lanes at W=12: three LDS planes and 25 neighbours keep register
-// pressure high).
+// Tile shapes from sweeps at 512x256x256 (cold caches, 40-launch means), with
+// the prefetch ring kept in flight (rotating slots, buffer loads): laplacian
+// W=8/D=2/KC=8 101 us (KC=16 104, W=16 113); wave13pt W=12/D=3 141 us (W=16
+// 148, W=14 169); lapgsrb with 8-byte lanes, W=12/D=2 137 us (W=16 157; with
+// 16-byte lanes the 25-point body needs too many registers: 307 us).
 //                 VEC W   H  LL LR ZL ZR D  NAUX KC
-template <int V> using LapCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 16>;
-template <int V> using WaveCfg = kgp::Cfg<V, 16, 2, 0, 0, 2, 2, 2, 1, 32>;
-template <int V> using GsrbCfg = kgp::Cfg<1, 16, 2, 1, 1, 2, 2, 2, 0, 32>;
+template <int V> using LapCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 8>;
+template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
+template <int V> using GsrbCfg = kgp::Cfg<1, 12, 2, 1, 1, 2, 2, 2, 0, 32>;
 
 // Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx.
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, class Fill>
