@@ -183,6 +183,29 @@ int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, const void*
  * counter-based hash of (seed, index); asynchronous on the context stream. */
 int kg_fill_uniform(kg_ctx* ctx, int dtype, size_t n, void* p, unsigned long long seed);
 
+/* ---- 2D Poisson solve (mixed precision) ------------------------------------------
+ * A u = f on the unit square, u = 0 on the boundary, (n+1) x (n+1) vertices
+ * (n cells per side, n a power of two up to 16384, h = 1/n), x fastest.  A is
+ * the 9-point operator whose damped Jacobi sweep has the form of the
+ * reference's jacobi kernel (jacobi/jacobi.F90:65-67) plus a right-hand side:
+ * (A u) = (20 u - 4 sum_axis - sum_diag) / (6 h^2).  Defect correction: the
+ * FP64 residual r = f - A u, then `vcycles` FP32 multigrid V-cycles on A e = r,
+ * u += e, until ||r||_2 / ||f||_2 < tol or max_outer corrections.  u (device,
+ * FP64) holds the initial guess and receives the solution; f is not changed.
+ * Returns KG_ERR_STATE if tol was not reached. */
+#define KG_POISSON_HISTORY 32
+typedef struct kg_poisson_report {
+    int outer;                 /* corrections applied */
+    int vcycles;               /* FP32 V-cycles in total */
+    int levels;                /* multigrid levels */
+    double rel_residual;       /* final ||f - A u||_2 / ||f||_2 (FP64) */
+    double seconds;            /* device time of the solve */
+    double history[KG_POISSON_HISTORY];   /* relative residual before each correction */
+} kg_poisson_report;
+
+int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, double tol, int max_outer,
+                     int vcycles, kg_poisson_report* report);
+
 /* FP64 statistics of a device field x of n values of type dtype, optionally
  * against a reference field ref of type ref_dtype (NULL: none).  out (host,
  * 5 doubles) = { sum x, sum x^2, max |x|, sum (x-ref)^2, max |x-ref| }.

@@ -16,6 +16,9 @@ for.  Every function takes the kgstencil module (``kg``) and an open
 * ``run_precision``: laplacian and wave13pt at 512^3 for 1000 steps in FP64,
   FP32 and BF16 storage, with errors of the FP32/BF16 fields against the FP64
   run (bit-identical to the oracle) and FP64 norms/energy.
+* ``run_poisson``: 2D Poisson on 8192^2 cells with a random right-hand side,
+  FP32 multigrid sweeps inside FP64 defect correction to a relative residual
+  of 1e-10.
 """
 from __future__ import annotations
 
@@ -210,3 +213,28 @@ def run_precision(kg, ctx, n: int = 512, steps: int = 1000, check_every: int = 2
                 buf.free()
         out[kind] = {"coef": co, "checks": checks, "timing": timing}
     return out
+
+
+# ------------------------------------------------------------------ Poisson
+def run_poisson(kg, ctx, n: int = 8192, tol: float = 1e-10, seed: int = 11,
+                vcycles: int = 2) -> dict:
+    """2D Poisson on n x n cells ((n+1)^2 vertices), random right-hand side
+    uniform in [-1, 1) (zero on the boundary), FP32 multigrid V-cycles inside
+    FP64 defect correction to ||f - A u|| / ||f|| < tol."""
+    N = n + 1
+    f = ctx.empty((N * N,), np.float64)
+    u = ctx.empty((N * N,), np.float64)
+    kg._check(ctx.lib, ctx.lib.kg_fill_uniform(ctx.ptr, kg.F64, N * N, f.ptr, seed), "fill")
+    # zero the boundary rows/columns of f and the initial guess
+    fh = f.download().reshape(N, N)
+    fh[0, :] = fh[-1, :] = fh[:, 0] = fh[:, -1] = 0
+    f.upload(fh)
+    u.upload(np.zeros((N, N)))
+    t0 = time.perf_counter()
+    rep = kg.poisson_solve(ctx, n, f, u, tol=tol, vcycles=vcycles)
+    wall = time.perf_counter() - t0
+    f.free()
+    u.free()
+    rep.update({"n_cells": n, "vertices": N * N, "tol": tol, "wall_s": wall,
+                "vcycles_per_correction": vcycles})
+    return rep

@@ -267,7 +267,7 @@ _BATCH_SIGS = {
     "kg_batch_fill_grid": [_dp, _i, C.c_ulonglong],
     "kg_batch_step_sequential": [_dp, _i],
 }
-EXPORTS += list(_BATCH_SIGS) + ["kg_serve_run"]
+EXPORTS += list(_BATCH_SIGS) + ["kg_serve_run", "kg_poisson_solve"]
 
 
 def _declare_batch(lib):
@@ -389,6 +389,26 @@ class Batch:
         _check(self.lib, self.lib.kg_batch_memory(self.ptr, C.byref(r), C.byref(u)),
                "kg_batch_memory")
         return r.value, u.value
+
+
+# ---------------------------------------------------------------- Poisson
+class PoissonReport(C.Structure):
+    _fields_ = [("outer", _i), ("vcycles", _i), ("levels", _i), ("rel_residual", _d),
+                ("seconds", _d), ("history", _d * 32)]
+
+
+def poisson_solve(ctx: Context, n: int, f: DeviceArray, u: DeviceArray, tol: float = 1e-10,
+                  max_outer: int = 20, vcycles: int = 2) -> dict:
+    """kg_poisson_solve: FP32 multigrid inside FP64 defect correction."""
+    fn = ctx.lib.kg_poisson_solve
+    fn.argtypes = [_dp, _i, _dp, _dp, _d, _i, _i, C.POINTER(PoissonReport)]
+    fn.restype = _i
+    rep = PoissonReport()
+    _check(ctx.lib, fn(ctx.ptr, n, f.ptr, u.ptr, tol, max_outer, vcycles, C.byref(rep)),
+           "kg_poisson_solve")
+    return {"outer": rep.outer, "vcycles": rep.vcycles, "levels": rep.levels,
+            "rel_residual": rep.rel_residual, "seconds": rep.seconds,
+            "history": list(rep.history[: min(rep.outer + 1, 32)])}
 
 
 # ---------------------------------------------------------------- serving
