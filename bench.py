@@ -16,7 +16,13 @@ read once, each output written once) x points / mean launch duration, against
 (oracle/_ref/libref_fast.so: compiled from the reference sources with the
 reference's gcc flags) on one host core over a bounded sample.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+The line also carries `layers`: the batched, served, precision and Poisson
+layers beside the FP64 drop-in, each at its own configuration (4096 grids x
+100 steps FP64 and FP32; Poisson-arrival serving at several loads; 512^3 x
+1000 steps FP64/FP32/BF16 with errors against FP64; Poisson 8192^2 to 1e-10),
+and the uint8 Life throughput at 8192^2.  `--no-layers` skips them.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-layers]
 """
 from __future__ import annotations
 
@@ -210,6 +216,74 @@ def cpu_leg(reps: int, rng):
     }
 
 
+def load_layers():
+    spec = importlib.util.spec_from_file_location("kglayers", PKG / "kglayers.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def life_leg(kg, ctx, n: int = 8192, reps: int = 20) -> dict:
+    """uint8 Life (gameoflife X mode), n x n cells, Bernoulli p = 0.3."""
+    rng = np.random.default_rng(5)
+    board = (rng.random((n, n)) < 0.3).astype(np.uint8)
+    a, b = ctx.array(board), ctx.array(board)
+    for _ in range(3):
+        ctx.call("life_u8", n, n, a, b)
+        a, b = b, a
+    ctx.sync()
+    ctx.drain()
+    for _ in range(reps):
+        ctx.call("life_u8", n, n, a, b)
+        a, b = b, a
+    t = float(np.mean(ctx.drain()))
+    a.free()
+    b.free()
+    return {"cells": n * n, "us": round(t * 1e6, 2), "gcells_s": round(n * n / t / 1e9, 1),
+            "gbs": round(2 * n * n / t / 1e9, 1), "frac_hbm": round(2 * n * n / t / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def layers_leg(kg, suite_per: dict) -> dict:
+    """The layers beside the drop-in kernels, summarised for the JSON line."""
+    L = load_layers()
+    ctx = kg.Context(0)
+    out = {}
+    try:
+        out["life_u8_8192"] = life_leg(kg, ctx)
+        plain = suite_per.get("laplacian", {}).get("gpts_s")
+        for name, dt in (("batch_f64", kg.F64), ("batch_f32", kg.F32)):
+            r = L.run_batch(kg, ctx, dt)
+            out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        if plain:
+            out["batch_f32_vs_plain_fp64_laplacian_512x256x256"] = round(
+                out["batch_f32"]["gpts_s"] / plain, 3)
+            out["batch_f32_vs_sequential_fp64_launches"] = round(
+                out["batch_f32"]["gpts_s"] / out["batch_f64"]["sequential_gpts_s"], 2)
+        s = L.run_serve(kg, ctx)
+        out["serve"] = {
+            "jobs": s["jobs"], "budget_bytes": s["budget_bytes"],
+            "capacity_jobs_s": round(s["capacity_jobs_s"], 1),
+            "loads": [{"offered_load": r["offered_load"],
+                       **{m: {k: round(r[m][k], 4) for k in ("p50_s", "p99_s", "gpts_s")}
+                          for m in ("continuous", "sequential")},
+                       "peak_active": r["continuous"]["peak_active"]} for r in s["loads"]]}
+        p = L.run_precision(kg, ctx)
+        out["precision_512cubed_1000_steps"] = {
+            kind: {"final": {d: {k: float("%.3g" % v) for k, v in p[kind]["checks"][-1][d].items()}
+                             for d in ("f32", "bf16")},
+                   "gpts_s": {d: round(p[kind]["timing"][d]["gpts_s"], 1) for d in p[kind]["timing"]},
+                   "coef": p[kind]["coef"]}
+            for kind in ("laplacian", "wave13pt")}
+        q = L.run_poisson(kg, ctx)
+        out["poisson_8192"] = {k: q[k] for k in ("outer", "vcycles", "levels", "rel_residual",
+                                                 "seconds")}
+    except Exception as e:   # a layer failure is reported, not hidden
+        out["error"] = f"{type(e).__name__}: {e}"
+    finally:
+        ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +295,8 @@ def main() -> int:
                     help="also time a plain 512 MiB device copy (reported as copy_calibration)")
     ap.add_argument("--only", default="",
                     help="comma-separated subset of stencils (profiling runs)")
+    ap.add_argument("--no-layers", action="store_true",
+                    help="skip the batch / serving / precision / Poisson / uint8 Life layers")
     args = ap.parse_args()
     if args.only:
         keep = args.only.split(",")
@@ -270,8 +346,17 @@ def main() -> int:
     }
     if cal:
         line["copy_calibration"] = cal
+    if "tricubic" in per:
+        # tricubic is bound by FP64 issue: 252 multiplies and adds per point
+        # (no contraction), against 39.3 T non-FMA FP64 ops/s (78.6 TFLOP/s
+        # counts an FMA as two)
+        t = per["tricubic"]["us"] * 1e-6
+        per["tricubic"]["dp_ops_per_pt"] = 252
+        per["tricubic"]["frac_fp64_issue"] = round(252 * N / t / 39.3e12, 4)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg(args.cpu_reps, rng)
+    if not args.no_layers and not args.only:
+        line["layers"] = layers_leg(kg, per)
     print(json.dumps(line), flush=True)
     return 0
 

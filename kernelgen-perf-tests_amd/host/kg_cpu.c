@@ -8,6 +8,7 @@
  * lives in kg_cpu_whispering.c because its reference target is built -O0.
  */
 #include <malloc.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -365,5 +366,37 @@ int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, const double
                         A[(size_t)i + (size_t)nx * k] * B[(size_t)k + (size_t)ny * j];
         toc(ctx, "matmul_", t0_);
     }
+    return KG_OK;
+}
+
+/* ---- the suite's single-precision tests (real = float) ---- */
+int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const float* w0, const float* w1, float* w2)
+{
+    const double t0_ = now();
+    const size_t n = (size_t)nx * ny * ns;
+    for (size_t i = 0; i < n; i++) w2[i] = w0[i] + w1[i];
+    toc(ctx, "vecadd", t0_);
+    return KG_OK;
+}
+
+/* matvec/matvec.c:38-70 */
+int kg_matvec(kg_ctx* ctx, int nx, int ny, const float* A, const float* x, float* y)
+{
+    const double t0_ = now();
+    for (int j = 0; j < ny; j++) {
+        y[j] = 0;
+        for (int i = 0; i < nx; i++) y[j] += A[(size_t)i + (size_t)nx * j] * x[i];
+    }
+    toc(ctx, "matvec", t0_);
+    return KG_OK;
+}
+
+/* sincos/sincos.F90:62-70 */
+int kg_sincos(kg_ctx* ctx, int nx, int ny, int ns, const float* x, const float* y, float* xy)
+{
+    const double t0_ = now();
+    const size_t n = (size_t)nx * ny * ns;
+    for (size_t i = 0; i < n; i++) xy[i] = sinf(x[i]) + cosf(y[i]);
+    toc(ctx, "sincos", t0_);
     return KG_OK;
 }

@@ -18,6 +18,11 @@ double kg_real_rand(void)
     return (((double)(rand() / (double)RAND_MAX) - 0.5) * 2);
 }
 
+float kg_real_rand_f(void)
+{
+    return (float)((((float)(rand() / (double)RAND_MAX)) - 0.5) * 2);
+}
+
 void kg_get_time(struct timespec* t)
 {
     clock_gettime(CLOCK_REALTIME, t);
@@ -159,6 +164,61 @@ double kg_sum(const double* a, size_t n)
     double s = 0.0;
     for (size_t i = 0; i < n; i++) s += a[i];
     return s;
+}
+
+float* kg_host_array_f(size_t n)
+{
+    float* p = (float*)memalign(KG_MEMALIGN, n * sizeof(float));
+    if (!p) {
+        printf("Error allocating memory for arrays: %p\n", (void*)p);
+        exit(1);
+    }
+    return p;
+}
+
+void kg_alloc_phase_f(kg_run* r, int n, float** dev, float* const* host, const size_t* len)
+{
+    if (!kg_target_has_device) {
+        for (int i = 0; i < n; i++) dev[i] = host[i];
+        return;
+    }
+    kg_phase_start(r);
+    for (int i = 0; i < n; i++) {
+        void* p = NULL;
+        KG_CALL(kg_alloc(r->ctx, &p, len[i] * sizeof(float)));
+        dev[i] = (float*)p;
+    }
+    double t = kg_phase_stop(r);
+    if (!r->no_timing) printf("device buffer alloc time = %f sec\n", t);
+}
+
+void kg_load_phase_f(kg_run* r, int n, float* const* dev, float* const* host, const size_t* len,
+                     double shown_bytes)
+{
+    if (!kg_target_has_device) return;
+    kg_phase_start(r);
+    for (int i = 0; i < n; i++) KG_CALL(kg_h2d(r->ctx, dev[i], host[i], len[i] * sizeof(float)));
+    double t = kg_phase_stop(r);
+    if (!r->no_timing) printf("data load time = %f sec (%f GB/sec)\n", t, shown_bytes / (t * GIB));
+}
+
+void kg_save_phase_f(kg_run* r, int n, float* const* host, float* const* dev, const size_t* len,
+                     double shown_bytes)
+{
+    if (!kg_target_has_device) return;
+    kg_phase_start(r);
+    for (int i = 0; i < n; i++) KG_CALL(kg_d2h(r->ctx, host[i], dev[i], len[i] * sizeof(float)));
+    double t = kg_phase_stop(r);
+    if (!r->no_timing) printf("data save time = %f sec (%f GB/sec)\n", t, shown_bytes / (t * GIB));
+}
+
+void kg_free_phase_f(kg_run* r, int n, float* const* dev)
+{
+    if (!kg_target_has_device) return;
+    kg_phase_start(r);
+    for (int i = 0; i < n; i++) KG_CALL(kg_free(r->ctx, dev[i]));
+    double t = kg_phase_stop(r);
+    if (!r->no_timing) printf("device buffer free time = %f sec\n", t);
 }
 
 void kg_run_end(kg_run* r)

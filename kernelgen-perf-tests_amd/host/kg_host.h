@@ -43,6 +43,9 @@ int kg_parse_arg(const char* name, const char* arg);
 
 /* ((rand() / RAND_MAX) - 0.5) * 2, glibc rand() with its default seed. */
 double kg_real_rand(void);
+/* The same with real = float: (((real)(rand() / (double)RAND_MAX) - 0.5) * 2)
+ * -- the quotient is rounded to float before the shift (matvec/matvec.c:81). */
+float kg_real_rand_f(void);
 
 /* Wall-clock timer (CLOCK_REALTIME, seconds). */
 void kg_get_time(struct timespec* t);
@@ -96,5 +99,15 @@ void kg_save_phase(kg_run* r, int n, double* const* host, double* const* dev, co
 void kg_free_phase(kg_run* r, int n, double* const* dev);
 /* Plain left-to-right sum, as the reference's mean loops are written. */
 double kg_sum(const double* a, size_t n);
+
+/* The same phases for the single-precision tests (real = float in the
+ * reference: vecadd, matvec, sincos); len[i] counts floats. */
+float* kg_host_array_f(size_t n);
+void kg_alloc_phase_f(kg_run* r, int n, float** dev, float* const* host, const size_t* len);
+void kg_load_phase_f(kg_run* r, int n, float* const* dev, float* const* host, const size_t* len,
+                     double shown_bytes);
+void kg_save_phase_f(kg_run* r, int n, float* const* host, float* const* dev, const size_t* len,
+                     double shown_bytes);
+void kg_free_phase_f(kg_run* r, int n, float* const* dev);
 
 #endif

@@ -16,7 +16,7 @@ kg_cpu_whispering.o: $(HOSTDIR)/kg_cpu_whispering.c ../../include/kg_stencil.h
 	$(SILENT)$(GCC_O0) -c $< -o $@
 
 $(TEST): ../$(TEST).c $(HOSTDIR)/kg_host.c $(HOSTDIR)/kg_host.h kg_cpu.o kg_cpu_whispering.o
-	$(SILENT)$(MAINCC) ../$(TEST).c $(HOSTDIR)/kg_host.c kg_cpu.o kg_cpu_whispering.o -o $@ -lrt
+	$(SILENT)$(MAINCC) ../$(TEST).c $(HOSTDIR)/kg_host.c kg_cpu.o kg_cpu_whispering.o -o $@ -lrt -lm
 
 clean:
 	$(SILENT)rm -f $(TEST) *.o

@@ -17,9 +17,12 @@ TARGETS = \
 	lapgsrb \
 	laplacian \
 	matmul \
+	matvec \
+	sincos \
 	tricubic \
 	tricubic2 \
 	uxx1 \
+	vecadd \
 	wave13pt \
 	whispering
 

@@ -66,7 +66,20 @@ for t in $FTESTS; do
     gcc "$OUT/fast/${t}_prog.o" "$OUT/fast/${t}_f.o" "$OUT/fast/timing.o" -o "$OUT/bin/$t" -lrt
 done
 
-gcc -shared -o "$OUT/libref_ieee.so" "$OUT"/ieee/*.o -lrt
+# Single-precision tests (real = float, as their gcc makefiles default):
+# matvec (C) and sincos (C main + Fortran kernel, amdflang).
+C32="-std=c99 -D_GNU_SOURCE -Dreal=float -fPIC -w"
+gcc $C32 $IEEE -I"$REF/matvec" -Dmain=kgref_main_matvec -c "$REF/matvec/matvec.c" -o "$OUT/ieee/matvec.o"
+gcc $C32 $FAST -I"$REF/matvec" "$REF/matvec/matvec.c" "$REF/matvec/timing.c" -o "$OUT/bin/matvec" -lrt
+amdflang -cpp -fPIC $IEEE -c "$REF/sincos/sincos.F90" -o "$OUT/ieee/sincos_f.o"
+amdflang -cpp -fPIC $FAST -c "$REF/sincos/sincos.F90" -o "$OUT/fast/sincos_f32_f.o"
+gcc $C32 $FAST -I"$REF/sincos" -c "$REF/sincos/main.c" -o "$OUT/fast/sincos_prog.o"
+gcc $C32 $FAST -c "$REF/sincos/timing.c" -o "$OUT/fast/sincos_timing_prog.o"
+gcc "$OUT/fast/sincos_prog.o" "$OUT/fast/sincos_f32_f.o" "$OUT/fast/sincos_timing_prog.o" \
+    -o "$OUT/bin/sincos" -lrt -lm
+rm -f "$OUT/fast/sincos_f32_f.o" "$OUT/fast/sincos_timing_prog.o"
+
+gcc -shared -o "$OUT/libref_ieee.so" "$OUT"/ieee/*.o -lrt -lm
 gcc -shared -o "$OUT/libref_fast.so" $(ls "$OUT"/fast/*.o | grep -v _prog.o) -lrt
 rm -rf "$OUT/ieee" "$OUT/fast"
 echo "build_ref.sh: built $OUT (march=$MARCH)"
