@@ -238,3 +238,36 @@ def run_poisson(kg, ctx, n: int = 8192, tol: float = 1e-10, seed: int = 11,
     rep.update({"n_cells": n, "vertices": N * N, "tol": tol, "wall_s": wall,
                 "vcycles_per_correction": vcycles})
     return rep
+
+
+def _main(argv=None) -> int:
+    """python3 kglayers.py {batch,serve,precision,poisson} [--dtype f32|f64]:
+    run one layer on device 0 and print its JSON (used for profiling)."""
+    import argparse
+    import importlib.util
+    import json
+    from pathlib import Path
+    ap = argparse.ArgumentParser()
+    ap.add_argument("layer", choices=["batch", "serve", "precision", "poisson"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--steps", type=int, default=100)
+    args = ap.parse_args(argv)
+    spec = importlib.util.spec_from_file_location("kgstencil", Path(__file__).with_name("kgstencil.py"))
+    kg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(kg)
+    with kg.Context(0) as ctx:
+        if args.layer == "batch":
+            r = run_batch(kg, ctx, kg.F64 if args.dtype == "f64" else kg.F32, steps=args.steps)
+        elif args.layer == "serve":
+            r = run_serve(kg, ctx)
+        elif args.layer == "precision":
+            r = run_precision(kg, ctx)
+        else:
+            r = run_poisson(kg, ctx)
+    print(json.dumps(r), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(_main())
