@@ -76,8 +76,12 @@ int kg_check_plane_bytes(long long points, int elem_bytes);
 // tile id so that XCD x receives the contiguous logical range
 // [x*n/8, (x+1)*n/8): neighbouring tiles (which share halo cache lines) then
 // sit in the same L2.  Ids past the last multiple of 8 map to themselves.
+#ifndef KG_XCD_SWIZZLE
+#define KG_XCD_SWIZZLE 1
+#endif
 __device__ __forceinline__ unsigned kg_xcd_swizzle(unsigned b, unsigned n)
 {
+    if (!KG_XCD_SWIZZLE) return b;
     const unsigned per = n >> 3;
     if (b < (per << 3)) return (b & 7u) * per + (b >> 3);
     return b;

@@ -370,14 +370,16 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
 }
 
 // Single-grid launch in the dense layout: one workgroup per tile, tiles
-// enumerated x fastest, then y, then z-chunk, through the XCD swizzle.
+// enumerated x fastest, then y, then z-chunk, through the XCD swizzle when
+// SWZ (halo rows then hit in the XCD's L2; without it the chip streams one
+// contiguous window, which suits kernels with few LDS-served neighbours).
 // z-interior [ZB0, ns-ZB1), by default the same as the xy-interior.
-template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI>
+template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI, bool SWZ = true>
 __global__ __launch_bounds__(64 * C::W)
 void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
 {
     __shared__ typename C::F lds[C::NLS][C::ROWS][C::LW];
-    unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
+    unsigned b = SWZ ? kg_xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     const int bx = (int)(b % (unsigned)ntx);
     b /= (unsigned)ntx;
     const int by = (int)(b % (unsigned)nty);

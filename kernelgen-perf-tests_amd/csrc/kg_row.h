@@ -98,13 +98,13 @@ struct View {
 
 // Op supplies src[NF], side[NAUX] and run(view, o, x, xmask) which computes
 // and stores the lane's VEC points of row j (o = j*nx).
-template <class C, int LO, int HI, class Op>
+template <class C, int LO, int HI, class Op, bool SWZ = true>
 __global__ __launch_bounds__(64 * C::W)
 void row_kernel(int nx, int ny, int ntx, Op op)
 {
     __shared__ double lds[C::NF * C::NLS * C::LW];
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    const unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
+    const unsigned b = SWZ ? kg_xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     const int bx = (int)(b % (unsigned)ntx);
     const int by = (int)(b / (unsigned)ntx);
     const int x0 = bx * C::TXW;
@@ -213,6 +213,97 @@ __device__ __forceinline__ void store(double* dst, long long o, int x, const dou
         for (int v = 0; v < VEC; v++)
             if (xmask[v]) kg_st(dst + o + x + v, r[v]);
     }
+}
+
+// ------------------------------------------------------------ direct march
+// For single-field 2D stencils (no side streams): no LDS and no ring.  A lane
+// owns points x, x+1 (x even) of PY consecutive rows j0 .. j0+PY-1 and loads
+// the rows j0-H .. j0+PY-1+H it needs straight from global memory, each as
+// three aligned 16-byte loads covering points x-2 .. x+3, all of them before
+// the first use; the PY output rows are then computed from registers.  The
+// x- and y-neighbour loads of adjacent lanes and workgroups hit in the
+// caches, so HBM still sees each input once.  With the workgroups in
+// hardware order (no XCD swizzle) the whole chip streams one contiguous
+// window of the grid, which measured faster than the LDS-staged engine for
+// jacobi, gameoflife and gaussblur at 512 x 65536.
+template <int PY_, int ROWS_, int H_>
+struct DCfg {
+    static constexpr int VEC = 2;
+    static constexpr int PY = PY_;       // output rows per lane
+    static constexpr int ROWS = ROWS_;   // waves per workgroup, stacked in y
+    static constexpr int H = H_;         // stencil radius (<= 2)
+    static constexpr int NR = PY + 2 * H;
+    static constexpr int TXW = 128;
+    static_assert(H >= 1 && H <= 2, "points x-2 .. x+3 cover a radius of 2");
+};
+
+// Output row R of the lane's window (row j = j0 + R).
+template <class C, int R>
+struct DView {
+    const double (&win)[C::NR][6];   // rows j0-H .., points x-2 .. x+3
+    int j;
+    __device__ __forceinline__ double at(int, int v, int dx, int dy) const
+    {
+        return win[R + C::H + dy][2 + v + dx];
+    }
+};
+
+template <class C>
+struct DGrid {
+    int ntx;
+    unsigned blocks;
+    bool empty;
+};
+
+template <class C>
+static inline DGrid<C> make_dgrid(int nx, int ny, int lo, int hi)
+{
+    DGrid<C> g;
+    const int nyi = ny - hi - lo, nxi = nx - hi - lo;
+    g.empty = nxi <= 0 || nyi <= 0;
+    g.ntx = (nx + C::TXW - 1) / C::TXW;
+    const int per = C::ROWS * C::PY;
+    g.blocks = g.empty ? 0u : (unsigned)g.ntx * (unsigned)((nyi + per - 1) / per);
+    return g;
+}
+
+// Requires an even nx and 16-byte aligned fields (checked by the host).
+template <class C, int LO, int HI, class Op>
+__global__ __launch_bounds__(64 * C::ROWS)
+void direct_kernel(int nx, int ny, int ntx, Op op)
+{
+    const int lane = (int)(threadIdx.x & 63), r = (int)(threadIdx.x >> 6);
+    const int bx = (int)(blockIdx.x % (unsigned)ntx);
+    const int by = (int)(blockIdx.x / (unsigned)ntx);
+    const int x = bx * C::TXW + 2 * lane;
+    const int j0 = LO + (by * C::ROWS + r) * C::PY;
+    if (x >= nx || j0 >= ny - HI) return;   // no barriers in this kernel
+    // The pairs at x-2 and x+2 are clamped into the row at the grid's
+    // x-edges; there they only feed points outside the interior.  Rows past
+    // ny-1 are clamped likewise (they feed rows >= ny-HI, never stored).
+    const int xl = x >= 2 ? x - 2 : x;
+    const int xr = x + 2 < nx ? x + 2 : x;
+    const kgp::gmem<double>* src = kgp::as_global(op.src[0]);
+    double win[C::NR][6];
+#pragma unroll
+    for (int q = 0; q < C::NR; q++) {
+        const kgp::gmem<double>* row = src + (long long)min(j0 - C::H + q, ny - 1) * nx;
+        const d2 a = *(const kgp::gmem<d2>*)(row + xl);
+        const d2 c = *(const kgp::gmem<d2>*)(row + x);
+        const d2 e = *(const kgp::gmem<d2>*)(row + xr);
+        win[q][0] = a.x;
+        win[q][1] = a.y;
+        win[q][2] = c.x;
+        win[q][3] = c.y;
+        win[q][4] = e.x;
+        win[q][5] = e.y;
+    }
+    const bool xm[2] = {x >= LO && x < nx - HI, x + 1 >= LO && x + 1 < nx - HI};
+    kg_static_for<C::PY>([&](auto rr) {
+        constexpr int R = decltype(rr)::value;
+        const int j = j0 + R;
+        if (j < ny - HI) op.run(DView<C, R>{win, j}, (long long)j * nx, x, xm);
+    });
 }
 
 }  // namespace kgr

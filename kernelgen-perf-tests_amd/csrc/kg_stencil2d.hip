@@ -1,9 +1,12 @@
 // kg_stencil2d.hip -- FP64 2D stencils of the suite: jacobi, gaussblur,
 // gameoflife, whispering (the reference runs them at nx x (ny*ns), e.g.
-// 512 x 65536), on the row-staged engine of kg_row.h: x-neighbours and
-// diagonal neighbours from LDS rows, the lane's own column from registers,
-// every load prefetched a few rows ahead.  Arithmetic is in the reference's
-// source order and the library is built with -ffp-contract=off.
+// 512 x 65536).  The single-field stencils (jacobi, gaussblur, gameoflife)
+// run on the direct row march of kg_row.h (every neighbour loaded through
+// the caches, a few output rows per lane); whispering, with five fields and
+// three side streams, on the row-staged engine: x-neighbours and diagonal
+// neighbours from LDS rows, the lane's own column from registers, every load
+// prefetched a few rows ahead.  Arithmetic is in the reference's source order
+// and the library is built with -ffp-contract=off.
 #include "kg_row.h"
 
 #include <type_traits>
@@ -170,6 +173,49 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, Fill fill)
     return go(std::integral_constant<int, 1>{});
 }
 
+// Direct march shapes (output rows per lane, waves per workgroup) from
+// sweeps at 512x65536 (cold caches, against the row engine's 113-117 us):
+// jacobi PY=4/4 waves 102 us, gameoflife 103 us (PY=8: 102 / 106);
+// gaussblur PY=8/4 waves 100 us (PY=4: 110, 2 waves: 100-118).
+#ifndef KG_J9_PY
+#define KG_J9_PY 4
+#endif
+#ifndef KG_J9_ROWS
+#define KG_J9_ROWS 4
+#endif
+#ifndef KG_G25_PY
+#define KG_G25_PY 8
+#endif
+#ifndef KG_G25_ROWS
+#define KG_G25_ROWS 4
+#endif
+using J9D = kgr::DCfg<KG_J9_PY, KG_J9_ROWS, 1>;
+using G25D = kgr::DCfg<KG_G25_PY, KG_G25_ROWS, 2>;
+
+// Single-field stencils: the direct march when nx is even and both fields
+// are 16-byte aligned, else the row engine.
+template <class DC, template <int> class CfgT, template <class> class OpT, int LO, int HI,
+          class Fill>
+static int launch_direct(kg_ctx* ctx, const char* name, int nx, int ny, const void* in,
+                         const void* out, Fill fill)
+{
+    if (nx % 2 != 0 || (((uintptr_t)in | (uintptr_t)out) & 15))
+        return launch_row<CfgT, OpT, LO, HI, 2>(ctx, name, nx, ny, fill);
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (int rc = kg_check_dims2(nx, ny)) return rc;
+    using Op = OpT<DC>;
+    Op op{};
+    fill(op);
+    const kgr::DGrid<DC> g = kgr::make_dgrid<DC>(nx, ny, LO, HI);
+    auto kern = kgr::direct_kernel<DC, LO, HI, Op>;
+    kg_register_kernel(name, (const void*)kern);
+    if (int rc = kg_launch_begin(ctx, name)) return rc;
+    if (!g.empty)
+        hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * DC::ROWS), 0, ctx->stream, nx, ny,
+                           g.ntx, op);
+    return kg_launch_end(ctx);
+}
+
 #define KG_REQUIRE(cond, msg) \
     do { if (!(cond)) return kg_set_error(KG_ERR_INVALID, "%s", msg); } while (0)
 
@@ -180,7 +226,7 @@ extern "C" int kg_jacobi(kg_ctx* ctx, int nx, int ny, double c0, double c1, doub
 {
     KG_REQUIRE(w0 && w1, "kg_jacobi: null buffer");
     KG_REQUIRE(w0 != w1, "kg_jacobi: w0 and w1 must be distinct");
-    return launch_row<J9Cfg, JacobiOp, 1, 1, 2>(ctx, "jacobi_", nx, ny, [&](auto& op) {
+    return launch_direct<J9D, J9Cfg, JacobiOp, 1, 1>(ctx, "jacobi_", nx, ny, w0, w1, [&](auto& op) {
         op.src[0] = w0;
         op.dst = w1;
         op.c0 = c0;
@@ -195,7 +241,7 @@ extern "C" int kg_gaussblur(kg_ctx* ctx, int nx, int ny, double s0, double s1, d
     KG_REQUIRE(w0 && w1, "kg_gaussblur: null buffer");
     KG_REQUIRE(w0 != w1, "kg_gaussblur: w0 and w1 must be distinct");
     const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
-    return launch_row<G25Cfg, GaussblurOp, 2, 2, 2>(ctx, "gaussblur", nx, ny, [&](auto& op) {
+    return launch_direct<G25D, G25Cfg, GaussblurOp, 2, 2>(ctx, "gaussblur", nx, ny, w0, w1, [&](auto& op) {
         op.src[0] = w0;
         op.dst = w1;
         op.f = f;
@@ -212,7 +258,7 @@ extern "C" int kg_gameoflife(kg_ctx* ctx, int nx, int ny, const double* u0, doub
 {
     KG_REQUIRE(u0 && u1, "kg_gameoflife: null buffer");
     KG_REQUIRE(u0 != u1, "kg_gameoflife: u0 and u1 must be distinct");
-    return launch_row<J9Cfg, GameoflifeOp, 1, 1, 2>(ctx, "gameoflife", nx, ny, [&](auto& op) {
+    return launch_direct<J9D, J9Cfg, GameoflifeOp, 1, 1>(ctx, "gameoflife", nx, ny, u0, u1, [&](auto& op) {
         op.src[0] = u0;
         op.dst = u1;
     });

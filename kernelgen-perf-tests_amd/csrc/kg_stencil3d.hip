@@ -34,7 +34,7 @@ void divergence_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double
                   const double* __restrict__ uy, const double* __restrict__ uz)
 {
     kg_ztile t;
-    if (!kg_ztile_get<TX, TY, KC, 1, 1>(nx, ny, ns, ntx, nty, t)) return;
+    if (!kg_ztile_get<TX, TY, KC, 1, 1, false>(nx, ny, ns, ntx, nty, t)) return;
     const long long sz = (long long)nx * ny;
     const int sy = nx;
     long long o = kg_off3(t.i, t.j, t.k0, nx, sz);
@@ -159,7 +159,9 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 template <int V> using GsrbCfg = kgp::Cfg<1, 12, 2, 1, 1, 2, 2, 2, 0, 32>;
 
 // Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx.
-template <template <int> class CfgT, template <class> class OpT, int LO, int HI, class Fill>
+// SWZ: XCD-contiguous tile order (measured per kernel at 512x256x256).
+template <template <int> class CfgT, template <class> class OpT, int LO, int HI, bool SWZ = true,
+          class Fill>
 static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
@@ -172,7 +174,7 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, F
         Op op{};
         fill(op);
         const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI);
-        auto kern = kgp::plane_kernel<C, LO, HI, Op>;
+        auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, SWZ>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
@@ -229,7 +231,7 @@ extern "C" int kg_gradient(kg_ctx* ctx, int nx, int ny, int ns, double alpha, do
     KG_REQUIRE(u && ux && uy && uz, "kg_gradient: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz && ux != uy && ux != uz && uy != uz,
                "kg_gradient: buffers must be distinct");
-    return launch_plane<LapCfg, GradientOp, 1, 1>(ctx, "gradient", nx, ny, ns, [&](auto& op) {
+    return launch_plane<LapCfg, GradientOp, 1, 1, false>(ctx, "gradient", nx, ny, ns, [&](auto& op) {
         op.src = u;
         op.ux = ux;
         op.uy = uy;
@@ -271,7 +273,7 @@ extern "C" int kg_lapgsrb(kg_ctx* ctx, int nx, int ny, int ns, double c0, double
 {
     KG_REQUIRE(w0 && w1, "kg_lapgsrb: null buffer");
     KG_REQUIRE(w0 != w1, "kg_lapgsrb: w0 and w1 must be distinct");
-    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2>(ctx, "lapgsrb", nx, ny, ns, [&](auto& op) {
+    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, false>(ctx, "lapgsrb", nx, ny, ns, [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.c0 = c0;

@@ -80,7 +80,7 @@ void tricubic_k(int nx, int ny, int ns, int ntx, int nty,
 {
     static_assert(HI >= PFA, "a/b/c look-ahead must stay inside the grid");
     kg_ztile t;
-    if (!kg_ztile_get<TX, TY, KC, LO, HI>(nx, ny, ns, ntx, nty, t)) return;
+    if (!kg_ztile_get<TX, TY, KC, LO, HI, false>(nx, ny, ns, ntx, nty, t)) return;
     const long long sz = (long long)nx * ny;
     const int sy = nx;
     long long o = kg_off3(t.i, t.j, t.k0, nx, sz);

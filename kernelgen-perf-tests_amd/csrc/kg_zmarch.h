@@ -5,8 +5,11 @@
 // (coalesced) axis: one wave64 covers 64 consecutive x points of one row.
 // Tiles are aligned to x = 0 / y = 0 and cover the whole plane; threads whose
 // (i, j) lies outside the interior return at once (no barriers are used).
-// The hardware workgroup id goes through kg_xcd_swizzle so that each XCD's L2
-// holds one contiguous band of tiles (consecutive y-tiles of one z-chunk).
+// With SWZ the hardware workgroup id goes through kg_xcd_swizzle so that each
+// XCD's L2 holds one contiguous band of tiles (consecutive y-tiles of one
+// z-chunk); without it consecutive tiles go to different XCDs and the whole
+// chip streams one contiguous window of the grid, which measured faster for
+// the kernels whose neighbours are few or already in registers.
 #pragma once
 
 #include "kg_internal.h"
@@ -17,10 +20,10 @@ struct kg_ztile {
 };
 
 // lo / hi: interior is [lo, n - hi) along every axis.
-template <int TX, int TY, int KC, int LO, int HI>
+template <int TX, int TY, int KC, int LO, int HI, bool SWZ = true>
 __device__ __forceinline__ bool kg_ztile_get(int nx, int ny, int ns, int ntx, int nty, kg_ztile& t)
 {
-    unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
+    unsigned b = SWZ ? kg_xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     const int bx = (int)(b % (unsigned)ntx);
     b /= (unsigned)ntx;
     const int by = (int)(b % (unsigned)nty);
