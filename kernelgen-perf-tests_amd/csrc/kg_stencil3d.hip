@@ -4,8 +4,11 @@
 // laplacian, gradient, wave13pt and lapgsrb run on the plane-staged engine
 // (kg_plane.h): xy-neighbours from LDS, z-neighbours from a register column,
 // every global load prefetched a few planes ahead.  divergence and uxx1 read
-// each of their inputs at only a few offsets along one axis and use the plain
-// z-march (kg_zmarch.h): a TX x TY column tile marched through KC planes.
+// each of their inputs at only a few offsets along one axis: with an even nx
+// and 16-byte aligned fields they run the direct form (two points and a few
+// planes per lane, every operand a 16-byte load through the caches, all
+// issued up front), otherwise the plain z-march (kg_zmarch.h): a TX x TY
+// column tile marched through KC planes, one point per thread.
 // Arithmetic is written in the reference's source order and the library is
 // built with -ffp-contract=off: results are bit-identical to an IEEE build of
 // the reference kernels.
@@ -52,6 +55,77 @@ void divergence_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double
     }
 }
 
+// Direct form (even nx, 16-byte aligned fields): a lane owns points x, x+1
+// (x even) of row j and PZ consecutive planes; every operand is a 16-byte
+// load through the caches (ux as the aligned pairs at x-2, x, x+2; uy at rows
+// j-1, j+1; uz at planes k0-1 .. k0+PZ), all issued before the first use.
+// Workgroups stay in hardware order.  Same expression as divergence_k.
+#ifndef KG_DIV_PZ
+#define KG_DIV_PZ 4
+#endif
+#ifndef KG_DIV_ROWS
+#define KG_DIV_ROWS 4
+#endif
+constexpr int DIV_PZ = KG_DIV_PZ, DIV_ROWS = KG_DIV_ROWS;
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ d2v ld2(const double* p)
+{
+    return *(const kgp::gmem<d2v>*)kgp::as_global(p);
+}
+
+__global__ __launch_bounds__(64 * DIV_ROWS)
+void divergence2_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double beta,
+                   double gamma, double* __restrict__ u, const double* __restrict__ ux,
+                   const double* __restrict__ uy, const double* __restrict__ uz)
+{
+    unsigned b = blockIdx.x;
+    const int bx = (int)(b % (unsigned)ntx);
+    b /= (unsigned)ntx;
+    const int by = (int)(b % (unsigned)nty);
+    const int bz = (int)(b / (unsigned)nty);
+    const int lane = (int)(threadIdx.x & 63), r = (int)(threadIdx.x >> 6);
+    const int x = bx * 128 + 2 * lane;
+    const int j = 1 + by * DIV_ROWS + r;
+    const int k0 = 1 + bz * DIV_PZ;
+    if (x >= nx || j >= ny - 1 || k0 >= ns - 1) return;
+    // pairs at x-2 / x+2 clamped into the row at the x-edges (they then feed
+    // boundary points only); planes past ns-2 clamped (never stored)
+    const int xl = x >= 2 ? x - 2 : x;
+    const int xr = x + 2 < nx ? x + 2 : x;
+    const long long sz = (long long)nx * ny;
+    const long long oj = (long long)j * nx;
+    d2v xm[DIV_PZ], xc[DIV_PZ], xp[DIV_PZ], ym[DIV_PZ], yp[DIV_PZ], z[DIV_PZ + 2];
+#pragma unroll
+    for (int p = 0; p < DIV_PZ; p++) {
+        const long long o = (long long)min(k0 + p, ns - 2) * sz + oj;
+        xm[p] = ld2(ux + o + xl);
+        xc[p] = ld2(ux + o + x);
+        xp[p] = ld2(ux + o + xr);
+        ym[p] = ld2(uy + o - nx + x);
+        yp[p] = ld2(uy + o + nx + x);
+    }
+#pragma unroll
+    for (int p = 0; p < DIV_PZ + 2; p++) z[p] = ld2(uz + (long long)min(k0 - 1 + p, ns - 1) * sz + oj + x);
+    const bool m0 = x >= 1 && x < nx - 1, m1 = x + 1 < nx - 1;
+#pragma unroll
+    for (int p = 0; p < DIV_PZ; p++) {
+        if (k0 + p >= ns - 1) break;
+        double res[2];
+        res[0] = alpha * (xc[p].y - xm[p].y) + beta * (yp[p].x - ym[p].x) + gamma * (z[p + 2].x - z[p].x);
+        res[1] = alpha * (xp[p].x - xc[p].x) + beta * (yp[p].y - ym[p].y) + gamma * (z[p + 2].y - z[p].y);
+        double* o = u + (long long)(k0 + p) * sz + oj + x;
+        if (m0 && m1) {
+            d2v t = {res[0], res[1]};
+            kg_st((d2v*)o, t);
+        } else {
+            if (m0) kg_st(o, res[0]);
+            if (m1) kg_st(o + 1, res[1]);
+        }
+    }
+}
+
 // --------------------------------------------------------------------- uxx1
 // uxx1/uxx1.c:95-105, interior [2,n-1)^3, dth = 1/nx (:70)
 __global__ __launch_bounds__(TX * TY)
@@ -87,6 +161,93 @@ void uxx1_k(int nx, int ny, int ns, int ntx, int nty, double c1, double c2, doub
         xz_m1 = xz_c;
         xz_c = xz_p1;
         o += sz;
+    }
+}
+
+// Direct uxx1 (even nx, 16-byte aligned fields): a lane owns points x, x+1
+// of row j and PZ planes; xx as the aligned pairs at x-2, x, x+2; xy at rows
+// j-2 .. j+1; d1 at rows j-1, j of planes k0-1 .. k0+PZ-1; xz at planes
+// k0-2 .. k0+PZ; u0 at the point.  Same expression as uxx1_k.
+#ifndef KG_UXX_PZ
+#define KG_UXX_PZ 4
+#endif
+#ifndef KG_UXX_ROWS
+#define KG_UXX_ROWS 4
+#endif
+constexpr int UXX_PZ = KG_UXX_PZ, UXX_ROWS = KG_UXX_ROWS;
+
+__global__ __launch_bounds__(64 * UXX_ROWS)
+void uxx1_2_k(int nx, int ny, int ns, int ntx, int nty, double c1, double c2, double dth,
+              const double* __restrict__ u0, double* __restrict__ u1,
+              const double* __restrict__ d1, const double* __restrict__ xx,
+              const double* __restrict__ xy, const double* __restrict__ xz)
+{
+    unsigned b = blockIdx.x;
+    const int bx = (int)(b % (unsigned)ntx);
+    b /= (unsigned)ntx;
+    const int by = (int)(b % (unsigned)nty);
+    const int bz = (int)(b / (unsigned)nty);
+    const int lane = (int)(threadIdx.x & 63), r = (int)(threadIdx.x >> 6);
+    const int x = bx * 128 + 2 * lane;
+    const int j = 2 + by * UXX_ROWS + r;
+    const int k0 = 2 + bz * UXX_PZ;
+    if (x >= nx || j >= ny - 1 || k0 >= ns - 1) return;
+    const int xl = x >= 2 ? x - 2 : x;
+    const int xr = x + 2 < nx ? x + 2 : x;
+    const long long sz = (long long)nx * ny;
+    const long long oj = (long long)j * nx;
+    d2v uc[UXX_PZ], xa[UXX_PZ], xc[UXX_PZ], xb[UXX_PZ], y2[UXX_PZ], y1[UXX_PZ], y0[UXX_PZ],
+        y3[UXX_PZ], dc[UXX_PZ + 1], dy[UXX_PZ + 1], z[UXX_PZ + 3];
+#pragma unroll
+    for (int p = 0; p < UXX_PZ; p++) {
+        const long long o = (long long)min(k0 + p, ns - 2) * sz + oj;
+        uc[p] = ld2(u0 + o + x);
+        xa[p] = ld2(xx + o + xl);
+        xc[p] = ld2(xx + o + x);
+        xb[p] = ld2(xx + o + xr);
+        y2[p] = ld2(xy + o - 2 * nx + x);
+        y1[p] = ld2(xy + o - nx + x);
+        y0[p] = ld2(xy + o + x);
+        y3[p] = ld2(xy + o + nx + x);
+    }
+#pragma unroll
+    for (int p = 0; p < UXX_PZ + 1; p++) {
+        const long long o = (long long)min(k0 - 1 + p, ns - 2) * sz + oj + x;
+        dc[p] = ld2(d1 + o);
+        dy[p] = ld2(d1 + o - nx);
+    }
+#pragma unroll
+    for (int p = 0; p < UXX_PZ + 3; p++) z[p] = ld2(xz + (long long)min(k0 - 2 + p, ns - 1) * sz + oj + x);
+    const bool m0 = x >= 2 && x < nx - 1, m1 = x + 1 >= 2 && x + 1 < nx - 1;
+#pragma unroll
+    for (int p = 0; p < UXX_PZ; p++) {
+        if (k0 + p >= ns - 1) break;
+        double res[2];
+#pragma unroll
+        for (int v = 0; v < 2; v++) {
+            // xx at x+v, x+v-1, x+v+1, x+v-2 from the pairs at x-2, x, x+2
+            const double xx0 = v ? xc[p].y : xc[p].x;
+            const double xxm1 = v ? xc[p].x : xa[p].y;
+            const double xxp1 = v ? xb[p].x : xc[p].y;
+            const double xxm2 = v ? xa[p].y : xa[p].x;
+            const double d = 0.25 * ((v ? dc[p + 1].y : dc[p + 1].x) + (v ? dy[p + 1].y : dy[p + 1].x) +
+                                     (v ? dc[p].y : dc[p].x) + (v ? dy[p].y : dy[p].x));
+            res[v] = (v ? uc[p].y : uc[p].x) + (dth / d) * (
+                c1 * (xx0 - xxm1) +
+                c2 * (xxp1 - xxm2) +
+                c1 * ((v ? y0[p].y : y0[p].x) - (v ? y1[p].y : y1[p].x)) +
+                c2 * ((v ? y3[p].y : y3[p].x) - (v ? y2[p].y : y2[p].x)) +
+                c1 * ((v ? z[p + 2].y : z[p + 2].x) - (v ? z[p + 1].y : z[p + 1].x)) +
+                c2 * ((v ? z[p + 3].y : z[p + 3].x) - (v ? z[p].y : z[p].x)));
+        }
+        double* o = u1 + (long long)(k0 + p) * sz + oj + x;
+        if (m0 && m1) {
+            d2v t = {res[0], res[1]};
+            kg_st((d2v*)o, t);
+        } else {
+            if (m0) kg_st(o, res[0]);
+            if (m1) kg_st(o + 1, res[1]);
+        }
     }
 }
 
@@ -156,7 +317,16 @@ struct LapgsrbOp {
 //                 VEC W   H  LL LR ZL ZR D  NAUX KC
 template <int V> using LapCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 8>;
 template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
-template <int V> using GsrbCfg = kgp::Cfg<1, 12, 2, 1, 1, 2, 2, 2, 0, 32>;
+#ifndef KG_GSRB_W
+#define KG_GSRB_W 12
+#endif
+#ifndef KG_GSRB_D
+#define KG_GSRB_D 2
+#endif
+#ifndef KG_GSRB_KC
+#define KG_GSRB_KC 32
+#endif
+template <int V> using GsrbCfg = kgp::Cfg<1, KG_GSRB_W, 2, 1, 1, 2, 2, KG_GSRB_D, 0, KG_GSRB_KC>;
 
 // Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx.
 // SWZ: XCD-contiguous tile order (measured per kernel at 512x256x256).
@@ -222,6 +392,17 @@ extern "C" int kg_divergence(kg_ctx* ctx, int nx, int ny, int ns, double alpha, 
 {
     KG_REQUIRE(u && ux && uy && uz, "kg_divergence: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz, "kg_divergence: output aliases an input");
+    if (nx % 2 == 0 && !(((uintptr_t)u | (uintptr_t)ux | (uintptr_t)uy | (uintptr_t)uz) & 15)) {
+        if (int rc = kg_check_ctx(ctx)) return rc;
+        if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+        const kg_zgrid g = kg_zgrid_make(nx, ny, ns, 1, 1, 128, DIV_ROWS, DIV_PZ);
+        kg_register_kernel("divergence", (const void*)divergence2_k);
+        if (int rc = kg_launch_begin(ctx, "divergence")) return rc;
+        if (!g.empty)
+            hipLaunchKernelGGL(divergence2_k, dim3(g.blocks), dim3(64 * DIV_ROWS), 0, ctx->stream,
+                               nx, ny, ns, g.ntx, g.nty, alpha, beta, gamma, u, ux, uy, uz);
+        return kg_launch_end(ctx);
+    }
     KG_LAUNCH3("divergence", divergence_k, 1, 1, alpha, beta, gamma, u, ux, uy, uz);
 }
 
@@ -265,6 +446,19 @@ extern "C" int kg_uxx1(kg_ctx* ctx, int nx, int ny, int ns, double c1, double c2
     KG_REQUIRE(u1 != u0 && u1 != d1 && u1 != xx && u1 != xy && u1 != xz,
                "kg_uxx1: u1 aliases an input");
     const double dth = 1. / nx;   // uxx1/uxx1.c:70
+    if (nx % 2 == 0 && !(((uintptr_t)u0 | (uintptr_t)u1 | (uintptr_t)d1 | (uintptr_t)xx |
+                          (uintptr_t)xy | (uintptr_t)xz) & 15)) {
+        if (int rc = kg_check_ctx(ctx)) return rc;
+        if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+        // rows/planes from 2: the grid helper counts tiles from 0, one spare
+        const kg_zgrid g = kg_zgrid_make(nx, ny, ns, 2, 1, 128, UXX_ROWS, UXX_PZ);
+        kg_register_kernel("uxx1", (const void*)uxx1_2_k);
+        if (int rc = kg_launch_begin(ctx, "uxx1")) return rc;
+        if (!g.empty)
+            hipLaunchKernelGGL(uxx1_2_k, dim3(g.blocks), dim3(64 * UXX_ROWS), 0, ctx->stream,
+                               nx, ny, ns, g.ntx, g.nty, c1, c2, dth, u0, u1, d1, xx, xy, xz);
+        return kg_launch_end(ctx);
+    }
     KG_LAUNCH3("uxx1", uxx1_k, 2, 1, c1, c2, dth, u0, u1, d1, xx, xy, xz);
 }
 
