@@ -58,9 +58,10 @@ constexpr int NKIND = 5;        // KG_KIND_LAPLACIAN, _WAVE13PT, _GAUSSBLUR3, _J
 constexpr int NTYPE = 2;        // KG_F64, KG_F32
 constexpr int NLPR = 3;         // plane-engine shapes: lanes per row 64, 32, 16
 constexpr int FLAT = NLPR;      // shape index of the flat (cache-served) body
-constexpr int NSHAPE = NLPR + 1;
+constexpr int PACK = NLPR + 1;  // shape index of packed whole rows
+constexpr int NSHAPE = NLPR + 2;
 constexpr int NVAR = NKIND * NTYPE * NSHAPE;
-constexpr int LPRS[NSHAPE] = {64, 32, 16, 0};   // 0: flat
+constexpr int LPRS[NSHAPE] = {64, 32, 16, 0, -1};   // API codes: 0 flat, -1 packed rows
 constexpr int KCS[] = {8, 16, 32, 64};
 constexpr int NKC = sizeof(KCS) / sizeof(KCS[0]);
 
@@ -134,7 +135,7 @@ using FlatCfg = kgp::Cfg<1, BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind<K>::Sh::LR
 
 // Shape facts of every variant, for the host (tile counts) and the LDS size.
 struct VarInfo {
-    int txw, ty, lo, hi, zlo, zhi, vec, esize;
+    int txw, ty, lo, hi, zlo, zhi, vec, esize, h;
     size_t lds;
 };
 
@@ -144,11 +145,15 @@ constexpr VarInfo var_info()
     using B = typename Kind<K>::B;
     if constexpr (LI == FLAT) {
         using C = FlatCfg<K, DT>;
-        return VarInfo{0, 0, B::LO, B::HI, B::ZLO, B::ZHI, 1, (int)sizeof(typename C::S), 0};
+        return VarInfo{0, 0, B::LO, B::HI, B::ZLO, B::ZHI, 1, (int)sizeof(typename C::S), C::H, 0};
+    } else if constexpr (LI == PACK) {
+        using C = VarCfg<K, DT, 64>;
+        return VarInfo{0, 0, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC, (int)sizeof(typename C::S),
+                       C::H, kgp::Packed<C>::LDS_BYTES};
     } else {
         using C = VarCfg<K, DT, LPRS[LI]>;
         return VarInfo{C::TXW, C::TY, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC,
-                       (int)sizeof(typename C::S), C::LDS_BYTES};
+                       (int)sizeof(typename C::S), C::H, C::LDS_BYTES};
     }
 }
 
@@ -162,9 +167,18 @@ constexpr VarInfo var_info()
 #define KG_FLAT_LIST(X) \
     X(0, 0) X(0, 1) X(1, 0) X(1, 1) X(2, 0) X(2, 1) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
 
-#define KG_VAR_INFO4(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
-                            var_info<K, DT, FLAT>(),
-constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO4)};
+#define KG_VAR_INFO5(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
+                            var_info<K, DT, FLAT>(), var_info<K, DT, PACK>(),
+constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO5)};
+
+// Rows of a packed tile (halo rows included) for a grid of variant v with
+// row pitch sy; 0 when the grid's rows do not pack (see kgp::pack_rows).
+inline int pack_rows_of(const VarInfo& v, long long sy)
+{
+    if (sy <= 0 || sy % v.vec) return 0;
+    const long long rt = (64LL * BW) / (sy / v.vec);
+    return rt > 2 * v.h ? (int)rt : 0;
+}
 
 constexpr size_t max_lds()
 {
@@ -213,9 +227,14 @@ __device__ __forceinline__ unsigned long long uni(unsigned long long v)
     const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
     return ((unsigned long long)hi << 32) | lo;
 }
-__device__ __forceinline__ const GridDesc* uni(const GridDesc* p)
+__device__ __forceinline__ long long uni(long long v)
 {
-    return (const GridDesc*)uni((unsigned long long)(uintptr_t)p);
+    return (long long)uni((unsigned long long)v);
+}
+template <class T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    return (T*)uni((unsigned long long)(uintptr_t)p);
 }
 
 // The op of grid g at batch step `step`: levels chosen from the grid's local
@@ -227,32 +246,36 @@ __device__ __forceinline__ typename Kind<K>::template Op<C> make_op(const GridDe
     using Op = typename Kind<K>::template Op<C>;
     using S = typename C::S;
     using F = typename C::F;
-    const long long ls = (long long)step - g.step0;
+    // Level pointers read back into scalar registers: the tile bodies build
+    // buffer resources from them, which must be workgroup-uniform (a resource
+    // in vector registers makes the compiler loop over the lanes per load).
+    const long long ls = uni((long long)step - g.step0);
+    void* const lev[3] = {uni(g.lev[0]), uni(g.lev[1]), uni(g.lev[2])};
     Op op{};
     if constexpr (K == KG_KIND_WAVE13PT) {
-        op.side[0] = (const S*)g.lev[ls % 3];
-        op.src = (const S*)g.lev[(ls + 1) % 3];
-        op.dst = (S*)g.lev[(ls + 2) % 3];
+        op.side[0] = (const S*)lev[ls % 3];
+        op.src = (const S*)lev[(ls + 1) % 3];
+        op.dst = (S*)lev[(ls + 2) % 3];
         op.m0 = (F)g.c[0];
         op.m1 = (F)g.c[1];
         op.m2 = (F)g.c[2];
     } else if constexpr (K == KG_KIND_LAPLACIAN) {
-        op.src = (const S*)g.lev[ls & 1];
-        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.src = (const S*)lev[ls & 1];
+        op.dst = (S*)lev[(ls + 1) & 1];
         op.alpha = (F)g.c[0];
         op.beta = (F)g.c[1];
     } else if constexpr (K == KG_KIND_JACOBI2) {
-        op.src = (const S*)g.lev[ls & 1];
-        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.src = (const S*)lev[ls & 1];
+        op.dst = (S*)lev[(ls + 1) & 1];
         op.c0 = (F)g.c[0];
         op.c1 = (F)g.c[1];
         op.c2 = (F)g.c[2];
     } else if constexpr (K == KG_KIND_LIFE2) {
-        op.src = (const S*)g.lev[ls & 1];
-        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.src = (const S*)lev[ls & 1];
+        op.dst = (S*)lev[(ls + 1) & 1];
     } else {
-        op.src = (const S*)g.lev[ls & 1];
-        op.dst = (S*)g.lev[(ls + 1) & 1];
+        op.src = (const S*)lev[ls & 1];
+        op.dst = (S*)lev[(ls + 1) & 1];
         op.f = (F)g.c[0];
         op.s0 = (F)g.c[1];
         op.s1 = (F)g.c[2];
@@ -277,7 +300,25 @@ __device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ 
     using B = typename Kind<K>::B;
     const Op op = make_op<K, C>(g, step);
     kgp::plane_tile<C, B::LO, B::HI, B::ZLO, B::ZHI, Op>(
-        (F(*)[C::ROWS][C::LW])batch_lds, g.nx, g.ny, g.ns, g.sy, g.sz, bx, by, bz, g.kc, op);
+        (F(*)[C::ROWS][C::LW])batch_lds, uni(g.nx), uni(g.ny), uni(g.ns), uni(g.sy), uni(g.sz), bx,
+        by, bz, uni(g.kc), op);
+}
+
+template <int K, int DT>
+__device__ __attribute__((noinline)) void run_pack(const GridDesc* __restrict__ gp_, int by_,
+                                                   int bz_, unsigned long long step_)
+{
+    const GridDesc& g = *uni(gp_);
+    const int by = uni(by_), bz = uni(bz_);
+    const unsigned long long step = uni(step_);
+    using C = VarCfg<K, DT, 64>;
+    using Op = typename Kind<K>::template Op<C>;
+    using F = typename C::F;
+    using B = typename Kind<K>::B;
+    const Op op = make_op<K, C>(g, step);
+    kgp::packed_tile<C, B::LO, B::HI, B::ZLO, B::ZHI, Op>((F*)batch_lds, uni(g.nx), uni(g.ny),
+                                                          uni(g.ns), uni(g.sy), uni(g.sz), by, bz,
+                                                          uni(g.kc), op);
 }
 
 // Flat body: the workgroup sweeps the interior points of planes
@@ -354,7 +395,8 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
             KG_VAR_LIST(KG_VAR_CASE)
 #undef KG_VAR_CASE
 #define KG_FLAT_CASE(K, DT) \
-    case var_id(K, DT, FLAT): run_flat<K, DT>(g, it.bz, step); break;
+    case var_id(K, DT, FLAT): run_flat<K, DT>(g, it.bz, step); break;                 \
+    case var_id(K, DT, PACK): run_pack<K, DT>(g, it.by, it.bz, step); break;
             KG_FLAT_LIST(KG_FLAT_CASE)
 #undef KG_FLAT_CASE
         default: break;
@@ -391,6 +433,14 @@ __global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
 void seq_flat_k(const GridDesc* __restrict__ gp, unsigned long long step)
 {
     run_flat<K, DT>(gp, (int)blockIdx.x, step);
+}
+
+template <int K, int DT>
+__global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
+void seq_pack_k(const GridDesc* __restrict__ gp, int nty, unsigned long long step)
+{
+    const unsigned b = kg_xcd_swizzle(blockIdx.x, gridDim.x);
+    run_pack<K, DT>(gp, (int)(b % (unsigned)nty), (int)(b / (unsigned)nty), step);
 }
 
 // Uniform [-1, 1) values from a counter-based hash of (seed, element index):
@@ -586,6 +636,12 @@ void cut_tiles(const HostGrid& g, int gi, int li, int kc, std::vector<Item>& out
         for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++) out.push_back(Item{gi, 0, 0, bz});
         return;
     }
+    if (li == PACK) {
+        const int ty = pack_rows_of(v, g.sy) - 2 * v.h;   // > 0: checked by build_work
+        for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++)
+            for (int by = 0; by < (nyi + ty - 1) / ty; by++) out.push_back(Item{gi, 0, by, bz});
+        return;
+    }
     const int ntx = (nx + v.txw - 1) / v.txw;
     const int nty = (nyi + v.ty - 1) / v.ty;
     const int nkc = (nzi + kc - 1) / kc;
@@ -673,6 +729,8 @@ int build_work(kg_batch* b, int only_class = -1, int li_force = 0, int kc_force 
         } else {
             shape_of(b, g, li, kc);
         }
+        // grids whose rows do not pack take the 64-lane rows instead
+        if (li == PACK && !pack_rows_of(VARS[var_id(g.spec.kind, g.spec.dtype, PACK)], g.sy)) li = 0;
         g.li = li;
         g.kc = kc;
         descs[gi] = make_desc(g, li, kc);
@@ -1002,7 +1060,8 @@ extern "C" int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int pl
         if (LPRS[q] == lanes_per_row) li = q;
     if (li < 0 || planes_per_tile < 1)
         return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: lanes per row must be 16, 32, "
-                                            "64 or 0 (flat) and planes per tile positive");
+                                            "64, 0 (flat) or -1 (packed rows) and planes per "
+                                            "tile positive");
     b->tuned[size_class(b->grids[id].spec)] = {li, planes_per_tile};
     b->dirty = true;
     drop_graphs(b);
@@ -1088,6 +1147,22 @@ extern "C" int kg_batch_step_sequential(kg_batch* b, int nsteps)
         break;
                     KG_FLAT_LIST(KG_SEQ_FLAT)
 #undef KG_SEQ_FLAT
+                default: break;
+                }
+                continue;
+            }
+            if (g.li == PACK) {
+                const GridDesc* gp = b->d_desc + gi;
+                const int ty = pack_rows_of(v, g.sy) - 2 * v.h;
+                const int nty = (nyi + ty - 1) / ty, nkc = (nzi + g.kc - 1) / g.kc;
+                switch (g.spec.kind * NTYPE + g.spec.dtype) {
+#define KG_SEQ_PACK(K, DT)                                                                      \
+    case K * NTYPE + DT:                                                                        \
+        hipLaunchKernelGGL((seq_pack_k<K, DT>), dim3((unsigned)(nty * nkc)), dim3(64 * BW), 0,  \
+                           b->ctx->stream, gp, nty, step);                                      \
+        break;
+                    KG_FLAT_LIST(KG_SEQ_PACK)
+#undef KG_SEQ_PACK
                 default: break;
                 }
                 continue;

@@ -238,6 +238,67 @@ struct View {
     __device__ __forceinline__ F side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
 };
 
+// The plane loop shared by the tile shapes, planes k0 .. k1-1.  load(k, slot,
+// want) fills a ring slot with plane k (want false: an empty resource, the
+// slot reads 0), load_aux(k, aslot, want) the side streams, stage(k, slot)
+// writes a slot to the LDS plane of k, compute(phase, k, ring, aux) runs the
+// stencil of plane k after the barrier.
+//
+// Register rings of NP = RS + 1 physical slots.  The plane loop is unrolled
+// NP times and each copy uses compile-time slot indices (phase P: logical
+// slot r is physical (P + r) % NP), so a prefetch stays in flight in its own
+// registers until it is used D planes later: no register copies, hence no
+// wait for loads issued in the same iteration.  The planes the ring and LDS
+// need are all < ns (the stencil reach ZHI <= the z-boundary width), so
+// slots whose load is skipped are never read.
+template <class C, class Load, class LoadAux, class Stage, class Compute>
+__device__ __forceinline__ void march(int k0, int k1, Load&& load, LoadAux&& load_aux,
+                                      Stage&& stage, Compute&& compute)
+{
+    using F = typename C::F;
+    Slot<F, C::VEC> ring[C::NP];
+    ASlot<F, C::VEC, C::NA> aux[C::NP];
+#pragma unroll
+    for (int r = 0; r < C::NP; r++) {
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) ring[r].v[v] = F(0);
+        ring[r].h = F(0);
+    }
+#pragma unroll
+    for (int r = 0; r < C::RS; r++) load(k0 - C::ZLO + r, ring[r], true);
+#pragma unroll
+    for (int d = 0; d < C::D; d++) load_aux(k0 + d, aux[d], true);
+    // LDS planes k0-LL .. k0+LR-1 (outside the ring when LL > ZL)
+#pragma unroll
+    for (int dz = -C::LL; dz < C::LR; dz++) {
+        if (dz >= -C::ZLO) {
+            stage(k0 + dz, ring[C::ZLO + dz]);
+        } else {
+            Slot<F, C::VEC> t;
+            t.h = F(0);
+#pragma unroll
+            for (int v = 0; v < C::VEC; v++) t.v[v] = F(0);
+            load(k0 + dz, t, true);
+            stage(k0 + dz, t);
+        }
+    }
+    auto plane = [&](auto phase, int k) {
+        constexpr int P = decltype(phase)::value;
+        stage(k + C::LR, ring[(P + C::ZLO + C::LR) % C::NP]);
+        const int kp = k + C::ZHI + C::D;   // plane entering the ring
+        load(kp, ring[(P + C::RS) % C::NP], kp <= k1 - 1 + C::ZHI);
+        load_aux(k + C::D, aux[(P + C::D) % C::NP], k + C::D < k1);
+        __syncthreads();
+        compute(phase, k, ring, aux);
+    };
+    int k = k0;
+    for (; k + C::NP <= k1; k += C::NP)
+        kg_static_for<C::NP>([&](auto p) { plane(p, k + decltype(p)::value); });
+    kg_static_for<C::NP - 1>([&](auto p) {
+        if (k + decltype(p)::value < k1) plane(p, k + decltype(p)::value);
+    });
+}
+
 // One tile (bx, by, bz) of a grid: xy-interior [LO, n-HI), z-interior
 // [ZB0, ns-ZB1), planes ZB0 + bz*kc .. +kc.  Every thread of the workgroup
 // calls it (it contains barriers).  Op supplies
@@ -292,14 +353,14 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
     // VEC > 1 every lane with x < nx reads VEC values: callers guarantee that
     // rows hold a multiple of VEC elements (nx % VEC == 0 in the dense
     // layout, padded rows otherwise).
-    auto load = [&](int k, Slot<F, C::VEC>& s, bool want = true) {
+    auto load = [&](int k, Slot<F, C::VEC>& s, bool want) {
         const __amdgpu_buffer_rsrc_t r = plane_rsrc(op.src, k, want);
         bload<T, C::VEC>(r, off_v, s.v);
         F h[1];
         bload<T, 1>(r, off_h, h);
         s.h = h[0];
     };
-    auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s, bool want = true) {
+    auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s, bool want) {
 #pragma unroll
         for (int a = 0; a < C::NAUX; a++) bload<T, C::VEC>(plane_rsrc(op.side[a], k, want), off_a, s.a[a]);
     };
@@ -311,61 +372,115 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
         else if (c >= C::LPR - C::H) L[row][C::H + C::TXW + (c - (C::LPR - C::H))] = s.h;
     };
 
-    // Register rings of NP = RS + 1 physical slots.  The plane loop is
-    // unrolled NP times and each copy uses compile-time slot indices (phase
-    // P: logical slot r is physical (P + r) % NP), so a prefetch stays in
-    // flight in its own registers until it is used D planes later: no register
-    // copies, hence no wait for loads issued in the same iteration.
-    Slot<F, C::VEC> ring[C::NP];
-    ASlot<F, C::VEC, C::NA> aux[C::NP];
-#pragma unroll
-    for (int r = 0; r < C::NP; r++) {
-#pragma unroll
-        for (int v = 0; v < C::VEC; v++) ring[r].v[v] = F(0);
-        ring[r].h = F(0);
-    }
-#pragma unroll
-    for (int r = 0; r < C::RS; r++) load(k0 - C::ZLO + r, ring[r]);
-#pragma unroll
-    for (int d = 0; d < C::D; d++) load_aux(k0 + d, aux[d]);
-    // LDS planes k0-LL .. k0+LR-1 (outside the ring when LL > ZL)
-#pragma unroll
-    for (int dz = -C::LL; dz < C::LR; dz++) {
-        if (dz >= -C::ZLO) {
-            stage(k0 + dz, ring[C::ZLO + dz]);
-        } else {
-            Slot<F, C::VEC> t;
-            t.h = F(0);
-#pragma unroll
-            for (int v = 0; v < C::VEC; v++) t.v[v] = F(0);
-            load(k0 + dz, t);
-            stage(k0 + dz, t);
-        }
-    }
     bool xmask[C::VEC];
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
-
-    // Plane k at phase P.  The planes the ring and LDS need are all < ns
-    // (the stencil reach ZHI <= ZB1), so slots whose load is skipped are
-    // never read.
-    auto plane = [&](auto phase, int k) {
+    march<C>(k0, k1, load, load_aux, stage, [&](auto phase, int k, const auto& ring, const auto& aux) {
         constexpr int P = decltype(phase)::value;
-        stage(k + C::LR, ring[(P + C::ZLO + C::LR) % C::NP]);
-        const int kp = k + C::ZHI + C::D;   // plane entering the ring
-        load(kp, ring[(P + C::RS) % C::NP], kp <= k1 - 1 + C::ZHI);
-        load_aux(k + C::D, aux[(P + C::D) % C::NP], k + C::D < k1);
-        __syncthreads();
         if (comp_row) {
             View<C, P> vw{(const F(*)[C::ROWS][C::LW])lds, ring, aux, k, row, c};
             op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
+    });
+}
+
+// ------------------------------------------------------------ packed rows
+// A tile of whole padded rows, for grids whose rows are narrower than a wave:
+// the workgroup's 64*W lanes cover RT = 64*W / (sy/VEC) consecutive rows of
+// the plane back to back, sy/VEC lanes per row, rows y0-H .. y0-H+RT-1, of
+// which the middle RT-2H are computed (the LPR-shaped tiles would leave the
+// lanes past a short row's end idle).  All neighbours along x lie in the
+// staged rows, so no x-halo is loaded.  Requires sy % VEC == 0 and RT > 2H
+// (pack_rows() below); the LDS plane slot holds 64*W*VEC values plus H guard
+// values on each side.
+template <class C>
+struct Packed {
+    static constexpr int SL = 64 * C::W * C::VEC;           // values per LDS slot
+    static constexpr int G = C::H;                           // guard values
+    static constexpr size_t LDS_BYTES = sizeof(typename C::F) * (C::NLS * SL + 2 * G);
+};
+
+// Rows per packed tile (halo rows included) for row pitch sy; 0 if the shape
+// does not fit.
+template <class C>
+__host__ __device__ inline int pack_rows(long long sy)
+{
+    if (sy <= 0 || sy % C::VEC) return 0;
+    const long long rt = (64LL * C::W) / (sy / C::VEC);
+    return rt > 2 * C::H ? (int)rt : 0;
+}
+
+template <class C, int P = 0>
+struct PView {
+    using F = typename C::F;
+    const F* lds;                              // guard offset applied
+    const Slot<F, C::VEC>* ring;               // [NP]
+    const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
+    int k, lb, sy;                             // plane, lane base in a slot, row pitch
+    __device__ __forceinline__ F at(int v, int dx, int dy, int dz) const
+    {
+        return lds[((k + dz) % C::NLS) * Packed<C>::SL + lb + dy * sy + v + dx];
+    }
+    __device__ __forceinline__ F col(int v, int dz) const
+    {
+        return ring[(P + C::ZLO + dz) % C::NP].v[v];
+    }
+    __device__ __forceinline__ F side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
+};
+
+// Tile (by, bz) of a grid in packed rows; the same contract as plane_tile.
+template <class C, int LO, int HI, int ZB0, int ZB1, class Op>
+__device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, int ns, long long sy,
+                                            long long sz, int by, int bz, int kc, const Op& op)
+{
+    using S = typename C::S;
+    using F = typename C::F;
+    using T = typename C::T;
+    constexpr int SL = Packed<C>::SL;
+    F* L0 = lds + Packed<C>::G;
+    const int lane = (int)threadIdx.x;
+    const int cpr = (int)(sy / C::VEC);              // lanes per row
+    const int RT = pack_rows<C>(sy);
+    const int row = lane / cpr;
+    const int x = (lane - row * cpr) * C::VEC;
+    const bool lane_ok = row < RT;
+    const int y0 = LO + by * (RT - 2 * C::H);
+    const int k0 = ZB0 + bz * kc;
+    const int k1 = min(k0 + kc, ns - ZB1);
+    const int j = y0 - C::H + row;
+    const bool row_ok = lane_ok && j < ny;           // j >= 0: y0 >= LO >= H
+    const bool comp_row = lane_ok && row >= C::H && row < RT - C::H && j < ny - HI;
+    const int lb = row * (int)sy + x;
+    const unsigned plane_bytes = (unsigned)(sz * (long long)sizeof(S));
+    const unsigned off_v = row_ok ? (unsigned)(((long long)j * sy + x) * (long long)sizeof(S)) : KG_OOB;
+    const unsigned off_a = comp_row ? off_v : KG_OOB;
+    auto plane_rsrc = [&](const S* base, int k, bool want) {
+        const bool ok = want && k >= 0 && k < ns;
+        return make_rsrc(base + (long long)(ok ? k : 0) * sz, ok ? plane_bytes : 0u);
     };
-    int k = k0;
-    for (; k + C::NP <= k1; k += C::NP)
-        kg_static_for<C::NP>([&](auto p) { plane(p, k + decltype(p)::value); });
-    kg_static_for<C::NP - 1>([&](auto p) {
-        if (k + decltype(p)::value < k1) plane(p, k + decltype(p)::value);
+    auto load = [&](int k, Slot<F, C::VEC>& s, bool want) {
+        bload<T, C::VEC>(plane_rsrc(op.src, k, want), off_v, s.v);
+    };
+    auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s, bool want) {
+#pragma unroll
+        for (int a = 0; a < C::NAUX; a++) bload<T, C::VEC>(plane_rsrc(op.side[a], k, want), off_a, s.a[a]);
+    };
+    auto stage = [&](int k, const Slot<F, C::VEC>& s) {
+        if (lane_ok) {
+            F* L = L0 + (k % C::NLS) * SL + lb;
+#pragma unroll
+            for (int v = 0; v < C::VEC; v++) L[v] = s.v[v];
+        }
+    };
+    bool xmask[C::VEC];
+#pragma unroll
+    for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
+    march<C>(k0, k1, load, load_aux, stage, [&](auto phase, int k, const auto& ring, const auto& aux) {
+        constexpr int P = decltype(phase)::value;
+        if (comp_row) {
+            PView<C, P> vw{L0, ring, aux, k, lb, (int)sy};
+            op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
+        }
     });
 }
 

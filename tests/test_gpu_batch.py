@@ -153,16 +153,30 @@ def test_batch_shapes_and_tuning_do_not_change_results(gpu_ctx, batch):
     specs = _mixed_specs(rng, 9, 5, 90)
     ids, init = _fill(batch, specs, rng)
     batch.step(1, graph=False)
-    for lpr in (16, 32, 64, 0):        # 0: the flat (cache-served) body
+    for lpr in (16, 32, 64, 0, -1):    # 0: the flat body, -1: packed rows
         for kc in (1, 7, 64):
             batch.set_shape(ids[0], lpr, kc)
             batch.set_shape(ids[4], lpr, kc)
             assert batch.shape(ids[0]) == (lpr, kc)
             batch.step(1, graph=True)
-    _check_all(gpu_ctx, batch, specs, ids, init, 13, "forced shapes")
+    _check_all(gpu_ctx, batch, specs, ids, init, 16, "forced shapes")
     assert batch.tune(reps=1) >= 1
     batch.step(2, graph=True)
-    _check_all(gpu_ctx, batch, specs, ids, init, 15, "after tuning")
+    _check_all(gpu_ctx, batch, specs, ids, init, 18, "after tuning")
+
+
+@pytest.mark.parametrize("kc", [3, 16])
+def test_batch_packed_rows_match_oracle(gpu_ctx, batch, kc):
+    """Every grid in packed-row tiles (whole rows back to back across the
+    workgroup): rows of 3..69 points, FP64 and FP32, all kinds."""
+    rng = np.random.default_rng(40 + kc)
+    specs = _mixed_specs(rng, 24) + [("wave13pt", F64, (5, 70, 6)), ("gaussblur3", F32, (69, 5, 3)),
+                                     ("laplacian", F32, (3, 3, 3))]
+    ids, init = _fill(batch, specs, rng)
+    for g in ids:
+        batch.set_shape(g, -1, kc)
+    batch.step(3, graph=True)
+    _check_all(gpu_ctx, batch, specs, ids, init, 3, f"packed rows kc={kc}")
 
 
 def test_batch_grids_join_and_leave(gpu_ctx, batch):
@@ -196,15 +210,20 @@ def test_batch_rejects_bad_specs(kgmod, batch):
         batch.remove(12345)
 
 
-def test_batch_fill_and_sequential_path(gpu_ctx, kgmod):
+@pytest.mark.parametrize("lpr", [None, -1])
+def test_batch_fill_and_sequential_path(gpu_ctx, kgmod, lpr):
     """kg_batch_fill is deterministic in the seed; the per-grid sequential
-    launches give the persistent kernel's results."""
+    launches give the persistent kernel's results (default shapes, and every
+    grid in packed rows)."""
     rng = np.random.default_rng(6)
     specs = _mixed_specs(rng, 9, 4, 60)
     outs = []
     for mode in ("batch", "sequential"):
         b = kgmod.Batch(gpu_ctx)
         ids = [b.add(k, d, dims, COEFS[k]) for k, d, dims in specs]
+        if lpr is not None:
+            for g in ids:
+                b.set_shape(g, lpr, 8)
         b.fill(1234)
         first = [b.download(g, 0) for g in ids]
         if mode == "batch":
