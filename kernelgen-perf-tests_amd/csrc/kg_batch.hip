@@ -300,7 +300,7 @@ __device__ __attribute__((noinline)) void run_tile(const GridDesc* __restrict__ 
     using B = typename Kind<K>::B;
     const Op op = make_op<K, C>(g, step);
     kgp::plane_tile<C, B::LO, B::HI, B::ZLO, B::ZHI, Op>(
-        (F(*)[C::ROWS][C::LW])batch_lds, uni(g.nx), uni(g.ny), uni(g.ns), uni(g.sy), uni(g.sz), bx,
+        (F*)batch_lds, uni(g.nx), uni(g.ny), uni(g.ns), uni(g.sy), uni(g.sz), bx,
         by, bz, uni(g.kc), op);
 }
 

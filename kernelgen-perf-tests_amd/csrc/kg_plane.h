@@ -162,7 +162,8 @@ struct Cfg {
     static constexpr int RPW = 64 / LPR;             // rows per wave
     static constexpr int ROWS = W * RPW;             // tile rows, halo rows included
     static constexpr int TXW = LPR * VEC;            // tile width in points
-    static constexpr int LW = TXW + 2 * H;
+    static constexpr int HL = (H + VEC - 1) / VEC;   // x-halo lanes per side
+    static constexpr int LQ = LPR + 2 * HL;          // LDS lane slots per row and component
     static constexpr int NLS = LL + LR + 2;          // LDS plane slots
     static constexpr int ZLO = ZL;                   // ring starts at k-ZL
     static constexpr int ZHI = (ZR > LR ? ZR : LR);  // ring reaches k+ZHI before prefetch
@@ -170,7 +171,7 @@ struct Cfg {
     static constexpr int NP = RS + 1;                // physical slots (one being filled)
     static constexpr int TY = ROWS - 2 * H;          // computed rows per tile
     static constexpr int NA = NAUX > 0 ? NAUX : 1;
-    static constexpr size_t LDS_BYTES = sizeof(F) * NLS * ROWS * LW;
+    static constexpr size_t LDS_BYTES = sizeof(F) * NLS * ROWS * VEC * LQ;
     static_assert(LPR == 64 || LPR == 32 || LPR == 16, "lanes per row");
     static_assert(LPR > 2 * H, "halo lanes of a row must not overlap");
     static_assert(TY > 0, "tile has no computed rows");
@@ -212,6 +213,15 @@ static inline Grid make_grid(int nx, int ny, int ns, int lo, int hi)
     return make_grid<C>(nx, ny, ns, lo, hi, lo, hi, C::KC);
 }
 
+// LDS layout of a staged plane: component-major, lane-contiguous.  Point
+// x0 + q*VEC + v of tile row r (lane q of the row holds its VEC points) sits at
+// [slot][r][v][HL + q], the x-halo points at lanes -HL .. -1 and LPR ..
+// LPR+HL-1.  A neighbour (v + dx) of a lane is then component (v+dx) mod VEC
+// of lane q + floor((v+dx)/VEC): consecutive lanes read consecutive words,
+// free of bank conflicts (a point-contiguous row, read at a stride of VEC
+// values per lane, put several lanes of a wave on one bank).
+__host__ __device__ constexpr int floor_div(int e, int d) { return (e + 8 * d) / d - 8; }
+
 // Per-lane view handed to a stencil's compute function at ring phase P: the
 // register ring's logical slot r (plane k - ZL + r) is physical slot
 // (P + r) % NP, and the side-stream ring's logical slot d (plane k + d) is
@@ -219,7 +229,7 @@ static inline Grid make_grid(int nx, int ny, int ns, int lo, int hi)
 template <class C, int P = 0>
 struct View {
     using F = typename C::F;
-    const F (*lds)[C::ROWS][C::LW];            // [NLS][ROWS][LW]
+    const F* lds;                              // [NLS][ROWS][VEC][LQ]
     const Slot<F, C::VEC>* ring;               // [NP]
     const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
     int k, row, cl;   // plane, tile row and lane-in-row of this lane
@@ -228,7 +238,8 @@ struct View {
     __device__ __forceinline__ F at(int v, int dx, int dy, int dz) const
     {
         const int s = (k + dz) % C::NLS;
-        return lds[s][row + dy][C::H + cl * C::VEC + v + dx];
+        const int lq = floor_div(v + dx, C::VEC), vq = v + dx - lq * C::VEC;
+        return lds[((s * C::ROWS + row + dy) * C::VEC + vq) * C::LQ + C::HL + cl + lq];
     }
     // main field at (x + v, j, k + dz) from registers, -ZL <= dz <= ZHI
     __device__ __forceinline__ F col(int v, int dz) const
@@ -309,7 +320,7 @@ __device__ __forceinline__ void march(int k0, int k1, Load&& load, LoadAux&& loa
 // j*sy, x the first point's column) and stores them; xmask[v] says whether
 // point x+v is inside the interior.
 template <class C, int LO, int HI, int ZB0, int ZB1, class Op>
-__device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW], int nx, int ny,
+__device__ __forceinline__ void plane_tile(typename C::F* lds, int nx, int ny,
                                            int ns, long long sy, long long sz, int bx, int by,
                                            int bz, int kc, const Op& op)
 {
@@ -365,11 +376,16 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
         for (int a = 0; a < C::NAUX; a++) bload<T, C::VEC>(plane_rsrc(op.side[a], k, want), off_a, s.a[a]);
     };
     auto stage = [&](int k, const Slot<F, C::VEC>& s) {
-        F(*L)[C::LW] = lds[k % C::NLS];
+        F* L = lds + ((k % C::NLS) * C::ROWS + row) * C::VEC * C::LQ;
 #pragma unroll
-        for (int v = 0; v < C::VEC; v++) L[row][C::H + c * C::VEC + v] = s.v[v];
-        if (c < C::H) L[row][c] = s.h;
-        else if (c >= C::LPR - C::H) L[row][C::H + C::TXW + (c - (C::LPR - C::H))] = s.h;
+        for (int v = 0; v < C::VEC; v++) L[v * C::LQ + C::HL + c] = s.v[v];
+        // the x-halo value of the first and last H lanes: tile column e < 0 or >= TXW
+        const bool lo_h = c < C::H, hi_h = c >= C::LPR - C::H;
+        if (lo_h || hi_h) {
+            const int e = lo_h ? c - C::H : C::TXW + (c - (C::LPR - C::H));
+            const int lq = floor_div(e, C::VEC);
+            L[(e - lq * C::VEC) * C::LQ + C::HL + lq] = s.h;
+        }
     };
 
     bool xmask[C::VEC];
@@ -378,7 +394,7 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
     march<C>(k0, k1, load, load_aux, stage, [&](auto phase, int k, const auto& ring, const auto& aux) {
         constexpr int P = decltype(phase)::value;
         if (comp_row) {
-            View<C, P> vw{(const F(*)[C::ROWS][C::LW])lds, ring, aux, k, row, c};
+            View<C, P> vw{lds, ring, aux, k, row, c};
             op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
     });
@@ -391,13 +407,11 @@ __device__ __forceinline__ void plane_tile(typename C::F (*lds)[C::ROWS][C::LW],
 // which the middle RT-2H are computed (the LPR-shaped tiles would leave the
 // lanes past a short row's end idle).  All neighbours along x lie in the
 // staged rows, so no x-halo is loaded.  Requires sy % VEC == 0 and RT > 2H
-// (pack_rows() below); the LDS plane slot holds 64*W*VEC values plus H guard
-// values on each side.
+// (pack_rows() below).
 template <class C>
 struct Packed {
-    static constexpr int SL = 64 * C::W * C::VEC;           // values per LDS slot
-    static constexpr int G = C::H;                           // guard values
-    static constexpr size_t LDS_BYTES = sizeof(typename C::F) * (C::NLS * SL + 2 * G);
+    static constexpr int NL = 64 * C::W + 2 * C::HL;         // lane slots per component
+    static constexpr size_t LDS_BYTES = sizeof(typename C::F) * C::NLS * C::VEC * NL;
 };
 
 // Rows per packed tile (halo rows included) for row pitch sy; 0 if the shape
@@ -410,16 +424,21 @@ __host__ __device__ inline int pack_rows(long long sy)
     return rt > 2 * C::H ? (int)rt : 0;
 }
 
+// LDS slot layout [NLS][VEC][NL], component-major and lane-contiguous as in
+// the LPR-shaped tiles: the lanes of a packed tile hold consecutive VEC-point
+// pieces of the row-major plane, so a neighbour (v+dx, dy) of lane L is
+// component (v+dx) mod VEC of lane L + floor((v+dx)/VEC) + dy*(sy/VEC).
 template <class C, int P = 0>
 struct PView {
     using F = typename C::F;
-    const F* lds;                              // guard offset applied
+    const F* lds;                              // guard lanes skipped
     const Slot<F, C::VEC>* ring;               // [NP]
     const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
-    int k, lb, sy;                             // plane, lane base in a slot, row pitch
+    int k, lane, cpr;                          // plane, lane, lanes per row
     __device__ __forceinline__ F at(int v, int dx, int dy, int dz) const
     {
-        return lds[((k + dz) % C::NLS) * Packed<C>::SL + lb + dy * sy + v + dx];
+        const int lq = floor_div(v + dx, C::VEC), vq = v + dx - lq * C::VEC;
+        return lds[(((k + dz) % C::NLS) * C::VEC + vq) * Packed<C>::NL + lane + lq + dy * cpr];
     }
     __device__ __forceinline__ F col(int v, int dz) const
     {
@@ -436,8 +455,8 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
     using S = typename C::S;
     using F = typename C::F;
     using T = typename C::T;
-    constexpr int SL = Packed<C>::SL;
-    F* L0 = lds + Packed<C>::G;
+    constexpr int NL = Packed<C>::NL;
+    F* L0 = lds + C::HL;
     const int lane = (int)threadIdx.x;
     const int cpr = (int)(sy / C::VEC);              // lanes per row
     const int RT = pack_rows<C>(sy);
@@ -450,7 +469,6 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
     const int j = y0 - C::H + row;
     const bool row_ok = lane_ok && j < ny;           // j >= 0: y0 >= LO >= H
     const bool comp_row = lane_ok && row >= C::H && row < RT - C::H && j < ny - HI;
-    const int lb = row * (int)sy + x;
     const unsigned plane_bytes = (unsigned)(sz * (long long)sizeof(S));
     const unsigned off_v = row_ok ? (unsigned)(((long long)j * sy + x) * (long long)sizeof(S)) : KG_OOB;
     const unsigned off_a = comp_row ? off_v : KG_OOB;
@@ -467,9 +485,9 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
     };
     auto stage = [&](int k, const Slot<F, C::VEC>& s) {
         if (lane_ok) {
-            F* L = L0 + (k % C::NLS) * SL + lb;
+            F* L = L0 + (k % C::NLS) * C::VEC * NL + lane;
 #pragma unroll
-            for (int v = 0; v < C::VEC; v++) L[v] = s.v[v];
+            for (int v = 0; v < C::VEC; v++) L[v * NL] = s.v[v];
         }
     };
     bool xmask[C::VEC];
@@ -478,7 +496,7 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
     march<C>(k0, k1, load, load_aux, stage, [&](auto phase, int k, const auto& ring, const auto& aux) {
         constexpr int P = decltype(phase)::value;
         if (comp_row) {
-            PView<C, P> vw{L0, ring, aux, k, lb, (int)sy};
+            PView<C, P> vw{L0, ring, aux, k, lane, cpr};
             op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
     });
@@ -493,7 +511,7 @@ template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI, bool SW
 __global__ __launch_bounds__(64 * C::W)
 void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
 {
-    __shared__ typename C::F lds[C::NLS][C::ROWS][C::LW];
+    __shared__ typename C::F lds[C::LDS_BYTES / sizeof(typename C::F)];
     unsigned b = SWZ ? kg_xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     const int bx = (int)(b % (unsigned)ntx);
     b /= (unsigned)ntx;
