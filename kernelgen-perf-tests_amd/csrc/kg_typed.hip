@@ -199,13 +199,35 @@ extern "C" int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, 
 }
 
 // ======================================================= precision variants
+// Tile shapes (waves, prefetch planes, planes per workgroup) of the FP32
+// kernels, from sweeps at 512^3 (cold caches): laplacian 12/3/16 220 us
+// (8/2/16 227, 8/3/8 221, 16/2/16 228); wave13pt 16/3/64 326 us (16/2/32
+// 343, 12/3/32 358).
+#ifndef KG_LF32_W
+#define KG_LF32_W 12
+#endif
+#ifndef KG_LF32_D
+#define KG_LF32_D 3
+#endif
+#ifndef KG_LF32_KC
+#define KG_LF32_KC 16
+#endif
+#ifndef KG_WF32_W
+#define KG_WF32_W 16
+#endif
+#ifndef KG_WF32_D
+#define KG_WF32_D 3
+#endif
+#ifndef KG_WF32_KC
+#define KG_WF32_KC 64
+#endif
 
 extern "C" int kg_laplacian_f32(kg_ctx* ctx, int nx, int ny, int ns, float alpha, float beta,
                                 const float* w0, float* w1)
 {
     KG_REQUIRE(w0 && w1, "kg_laplacian_f32: null buffer");
     KG_REQUIRE((const void*)w0 != (const void*)w1, "kg_laplacian_f32: w0 and w1 must be distinct");
-    return launch_typed<LaplacianOp, kgp::F32, 4, 8, 2, 16>(ctx, "laplacian_f32", nx, ny, ns,
+    return launch_typed<LaplacianOp, kgp::F32, 4, KG_LF32_W, KG_LF32_D, KG_LF32_KC>(ctx, "laplacian_f32", nx, ny, ns,
                                                             [&](auto& op) {
         op.src = w0;
         op.dst = w1;
@@ -233,7 +255,7 @@ extern "C" int kg_wave13pt_f32(kg_ctx* ctx, int nx, int ny, int ns, float m0, fl
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt_f32: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_f32: w2 aliases an input");
-    return launch_typed<Wave13ptOp, kgp::F32, 4, 16, 2, 32>(ctx, "wave13pt_f32", nx, ny, ns,
+    return launch_typed<Wave13ptOp, kgp::F32, 4, KG_WF32_W, KG_WF32_D, KG_WF32_KC>(ctx, "wave13pt_f32", nx, ny, ns,
                                                             [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;

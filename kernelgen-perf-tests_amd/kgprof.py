@@ -35,7 +35,7 @@ def short_name(full: str) -> str:
                      ("tricubic", r"tricubic_k<1, 2, false>|TricubicOp"),
                      ("laplacian", r"laplacian_k|LaplacianOp"), ("gradient", r"gradient_k|GradientOp"),
                      ("wave13pt", r"wave13pt_k|Wave13ptOp"), ("lapgsrb", r"lapgsrb_k|LapgsrbOp"),
-                     ("divergence", r"divergence_k"), ("uxx1", r"uxx1_k"),
+                     ("divergence", r"divergence2?_k"), ("uxx1", r"uxx1(_2)?_k"),
                      ("jacobi", r"jacobi_k|JacobiOp"), ("gaussblur", r"gaussblur_k|GaussblurOp"),
                      ("whispering", r"whispering_k|WhisperingOp"),
                      ("gameoflife", r"gameoflife_k|GameoflifeOp"),
