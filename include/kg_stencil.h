@@ -234,16 +234,23 @@ int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, const double
  * No reference counterpart (the reference runs one grid per process).  A batch
  * keeps many independent 3D grids resident in HBM and advances all of them by
  * one step per persistent launch; kg_batch_step(b, n, 1) captures n launches
- * in a HIP graph and replays it.  Each grid's result is bit-identical to
- * stepping it alone with the single-grid entry point of its kind and type
- * (kg_laplacian / _f32, kg_wave13pt / _f32, kg_gaussblur3_f64 / _f32).
+ * in a HIP graph and replays it.
  *
  * Levels: laplacian and gaussblur3 grids have 2 (ping-pong), wave13pt 3
  * (w0, w1, w2).  Before the first step, level 0 is the input (wave13pt: w0,
- * and level 1 is w1); upload every level, since boundary cells are never
- * written and keep what each level held.  kg_batch_newest_level tells which
- * level holds the latest values.  Host buffers are dense (x fastest);
- * device rows are padded to 16 bytes. */
+ * and level 1 is w1).  A grid's levels share one boundary: the cells outside
+ * the kind's interior box hold the same values in every level (a Dirichlet
+ * boundary).  kg_batch_fill writes them so; uploads must do the same.  A
+ * step writes whole rows of the output level -- the interior points and,
+ * copied from the input level, the x-boundary points of those rows -- so that
+ * no cache line of a row is written in part (partly written row-edge lines
+ * made narrow grids up to 2.3x slower); every other boundary cell is never
+ * written.  With a shared boundary, each grid's result is bit-identical to
+ * stepping it alone with the single-grid entry point of its kind and type
+ * (kg_laplacian / _f32, kg_wave13pt / _f32, kg_gaussblur3_f64 / _f32),
+ * whose boundary cells are never written.  kg_batch_newest_level tells which
+ * level holds the latest values.  Host buffers are dense (x fastest); device
+ * rows are padded to 16 bytes. */
 /* Kinds: the 3D stencils, and the 2D ones applied to every xy-plane (a 2D
  * nx x ny problem is a grid with ns = 1): jacobi (jacobi/jacobi.F90:65-67,
  * interior [1,n-1)) and gameoflife's FP rule (gameoflife/gameoflife.c:82-91). */
@@ -291,7 +298,8 @@ int kg_batch_grid_shape(kg_batch* b, int id, int* lanes_per_row, int* planes_per
 /* Forces the tile shape of the size class of grid id. */
 int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int planes_per_tile);
 /* Fills every level of every grid on the device with uniform [-1, 1) values
- * from a counter-based hash of (seed, grid, level, element). */
+ * from a counter-based hash: interior cells of (seed, grid, level, element),
+ * boundary cells of (seed, grid, element), shared by the levels. */
 int kg_batch_fill(kg_batch* b, unsigned long long seed);
 /* The comparison path: n steps as one plain launch per grid and step (same
  * tile bodies and results as kg_batch_step). */
@@ -299,7 +307,7 @@ int kg_batch_step_sequential(kg_batch* b, int nsteps);
 /* HBM reserved by the batch's arena and bytes held by live grids. */
 int kg_batch_memory(kg_batch* b, size_t* reserved, size_t* in_use);
 /* Fills the levels of one grid like kg_batch_fill (asynchronous); both levels
- * of a jacobi2 grid get the same values (one Dirichlet boundary). */
+ * of a jacobi2 grid get the same values throughout. */
 int kg_batch_fill_grid(kg_batch* b, int id, unsigned long long seed);
 
 /* ---- serving -------------------------------------------------------------------

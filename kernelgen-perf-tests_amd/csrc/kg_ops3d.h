@@ -32,7 +32,7 @@ struct LaplacianOp {
                 p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
                 p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
                 p.col(v, 1) + p.col(v, -1));
-        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+        p.put(dst, ob, x, r, xm);
     }
 };
 
@@ -62,7 +62,7 @@ struct Wave13ptOp {
                     p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
                     p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
                     p.col(v, 2) + p.col(v, -2));
-        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+        p.put(dst, ob, x, r, xm);
     }
 };
 
@@ -95,7 +95,7 @@ struct Gaussblur3Op {
                 s8 * (W(-2, -2) + W(2, -2) + W(-2, 2) + W(2, 2)));
 #undef W
         }
-        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+        p.put(dst, ob, x, r, xm);
     }
 };
 
@@ -120,7 +120,7 @@ struct Jacobi2Op {
             r[v] = c0 * p.at(v, 0, 0, 0) +
                    c1 * (p.at(v, -1, 0, 0) + p.at(v, 0, -1, 0) + p.at(v, 1, 0, 0) + p.at(v, 0, 1, 0)) +
                    c2 * (p.at(v, -1, -1, 0) + p.at(v, -1, 1, 0) + p.at(v, 1, -1, 0) + p.at(v, 1, 1, 0));
-        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+        p.put(dst, ob, x, r, xm);
     }
 };
 
@@ -148,7 +148,7 @@ struct Life2Op {
             r[v] = (F)1. / ((F)1. + (U(0, 0) + L - (F)3.) * (L - (F)3.) * Cbig);
 #undef U
         }
-        kgp::store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+        p.put(dst, ob, x, r, xm);
     }
 };
 
@@ -181,6 +181,11 @@ struct FlatView {
     }
     __device__ __forceinline__ F col(int, int dz) const { return kgp::ld_cvt<T>(src[o + dz * sz]); }
     __device__ __forceinline__ F side(int a, int) const { return kgp::ld_cvt<T>(aux[a][o]); }
+    template <class D>
+    __device__ __forceinline__ void put(D* dst, long long ob, int x, F (&r)[1], const bool (&xm)[1]) const
+    {
+        kgp::store<T, 1>(dst, ob, x, r, xm);
+    }
 };
 
 // Engine shape each op needs: halo width, LDS planes, register column and

@@ -143,7 +143,7 @@ __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned off,
 
 // ---------------------------------------------------------------- config
 template <int VEC_, int W_, int H_, int LL_, int LR_, int ZL_, int ZR_, int D_, int NAUX_, int KC_,
-          class T_ = F64, int LPR_ = 64>
+          class T_ = F64, int LPR_ = 64, bool FULL_ = false>
 struct Cfg {
     using T = T_;
     using S = typename T::S;           // storage type
@@ -159,6 +159,10 @@ struct Cfg {
     static constexpr int NAUX = NAUX_; // side streams read at the point only
     static constexpr int KC = KC_;     // planes per workgroup (single-grid launches)
     static constexpr int LPR = LPR_;   // lanes per row (64, 32 or 16)
+    // FULL: stores write every point of a computed row (x-boundary points get
+    // the centre value of the input level), so no cache line of the row is
+    // written in part; for fields whose levels share their boundary values.
+    static constexpr bool FULL = FULL_;
     static constexpr int RPW = 64 / LPR;             // rows per wave
     static constexpr int ROWS = W * RPW;             // tile rows, halo rows included
     static constexpr int TXW = LPR * VEC;            // tile width in points
@@ -233,6 +237,7 @@ struct View {
     const Slot<F, C::VEC>* ring;               // [NP]
     const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
     int k, row, cl;   // plane, tile row and lane-in-row of this lane
+    bool wr;          // the lane's points lie in the (padded) row
     // main field at (x + v + dx, j + dy, k + dz) from LDS, |dx|,|dy| <= H,
     // -LL <= dz <= LR
     __device__ __forceinline__ F at(int v, int dx, int dy, int dz) const
@@ -241,6 +246,10 @@ struct View {
         const int lq = floor_div(v + dx, C::VEC), vq = v + dx - lq * C::VEC;
         return lds[((s * C::ROWS + row + dy) * C::VEC + vq) * C::LQ + C::HL + cl + lq];
     }
+    // store the lane's results (see Cfg::FULL)
+    template <class S>
+    __device__ __forceinline__ void put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                        const bool (&xm)[C::VEC]) const;
     // main field at (x + v, j, k + dz) from registers, -ZL <= dz <= ZHI
     __device__ __forceinline__ F col(int v, int dz) const
     {
@@ -394,7 +403,7 @@ __device__ __forceinline__ void plane_tile(typename C::F* lds, int nx, int ny,
     march<C>(k0, k1, load, load_aux, stage, [&](auto phase, int k, const auto& ring, const auto& aux) {
         constexpr int P = decltype(phase)::value;
         if (comp_row) {
-            View<C, P> vw{lds, ring, aux, k, row, c};
+            View<C, P> vw{lds, ring, aux, k, row, c, x < sy};
             op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
     });
@@ -435,11 +444,15 @@ struct PView {
     const Slot<F, C::VEC>* ring;               // [NP]
     const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
     int k, lane, cpr;                          // plane, lane, lanes per row
+    bool wr;                                   // always true: packed lanes lie in their row
     __device__ __forceinline__ F at(int v, int dx, int dy, int dz) const
     {
         const int lq = floor_div(v + dx, C::VEC), vq = v + dx - lq * C::VEC;
         return lds[(((k + dz) % C::NLS) * C::VEC + vq) * Packed<C>::NL + lane + lq + dy * cpr];
     }
+    template <class S>
+    __device__ __forceinline__ void put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                        const bool (&xm)[C::VEC]) const;
     __device__ __forceinline__ F col(int v, int dz) const
     {
         return ring[(P + C::ZLO + dz) % C::NP].v[v];
@@ -496,7 +509,7 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
     march<C>(k0, k1, load, load_aux, stage, [&](auto phase, int k, const auto& ring, const auto& aux) {
         constexpr int P = decltype(phase)::value;
         if (comp_row) {
-            PView<C, P> vw{L0, ring, aux, k, lane, cpr};
+            PView<C, P> vw{L0, ring, aux, k, lane, cpr, true};
             op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
         }
     });
@@ -551,6 +564,40 @@ __device__ __forceinline__ void store(double* dst, long long ob, int x, const do
                                       const bool (&xmask)[VEC])
 {
     store<F64, VEC>(dst, ob, x, r, xmask);
+}
+
+// The views' store: interior points only, or with Cfg::FULL the lane's whole
+// vector, its non-interior points taking the input level's centre values.
+template <class C, class V, class S>
+__device__ __forceinline__ void put_row(const V& p, S* dst, long long ob, int x,
+                                        typename C::F (&r)[C::VEC], const bool (&xm)[C::VEC])
+{
+    if constexpr (C::FULL) {
+        if (!p.wr) return;
+        bool all[C::VEC];
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) {
+            if (!xm[v]) r[v] = p.col(v, 0);
+            all[v] = true;
+        }
+        store<typename C::T, C::VEC>(dst, ob, x, r, all);
+    } else {
+        store<typename C::T, C::VEC>(dst, ob, x, r, xm);
+    }
+}
+template <class C, int P>
+template <class S>
+__device__ __forceinline__ void View<C, P>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                                const bool (&xm)[C::VEC]) const
+{
+    put_row<C>(*this, dst, ob, x, r, xm);
+}
+template <class C, int P>
+template <class S>
+__device__ __forceinline__ void PView<C, P>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                                 const bool (&xm)[C::VEC]) const
+{
+    put_row<C>(*this, dst, ob, x, r, xm);
 }
 
 }  // namespace kgp

@@ -7,6 +7,12 @@ kg_wave13pt / _f32, kg_gaussblur3_f64 / _f32), which the parity tests pin to
 the oracle; FP64 grids are also checked against the oracle directly.  Graph
 replays, tuning, forced tile shapes and grids joining / leaving mid-run must
 not change any result.
+
+A batch grid's levels share their boundary values (kg_batch_fill writes them
+so; the uploads here do the same): a batch step writes whole rows of the
+output level, its x-boundary points copied from the input level, and with
+equal boundaries that is exactly the single-grid result, whose boundary
+points are never written.
 """
 import ctypes as C
 
@@ -27,12 +33,22 @@ SINGLE = {("laplacian", F64): "laplacian", ("laplacian", F32): "laplacian_f32",
           ("wave13pt", F64): "wave13pt", ("wave13pt", F32): "wave13pt_f32",
           ("gaussblur3", F64): "gaussblur3_f64", ("gaussblur3", F32): "gaussblur3_f32"}
 NLEV = {"laplacian": 2, "wave13pt": 3, "gaussblur3": 2}
+# interior box of each kind: x and y [lo, n-hi), z [zlo, ns-zhi)
+BOX = {"laplacian": (1, 1, 1, 1), "wave13pt": (2, 2, 2, 2), "gaussblur3": (2, 2, 0, 0)}
 
 
 def _levels(kind, dtype, dims, rng):
+    """Random levels whose cells outside the kind's interior box are equal
+    in every level."""
     nx, ny, ns = dims
     t = np.float64 if dtype == F64 else np.float32
-    return [rng.uniform(-1, 1, (ns, ny, nx)).astype(t) for _ in range(NLEV[kind])]
+    lv = [rng.uniform(-1, 1, (ns, ny, nx)).astype(t) for _ in range(NLEV[kind])]
+    lo, hi, zlo, zhi = BOX[kind]
+    inside = np.zeros((ns, ny, nx), bool)
+    inside[zlo:ns - zhi, lo:ny - hi, lo:nx - hi] = True
+    for a in lv[1:]:
+        a[~inside] = lv[0][~inside]
+    return lv
 
 
 def _single(ctx, kind, dtype, dims, levels, nsteps):

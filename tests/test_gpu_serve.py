@@ -99,6 +99,10 @@ def test_batch_2d_kinds_equal_dropin_per_plane(gpu_ctx, kgmod, kind):
             lv = [rng.uniform(-1, 1, (ns, ny, nx)) for _ in range(2)]
             if kind == "life2":
                 lv = [(a > 0.2).astype(np.float64) for a in lv]
+            # batch levels share their boundary (the x/y frame of every plane)
+            edge = np.ones((ns, ny, nx), bool)
+            edge[:, 1:ny - 1, 1:nx - 1] = False
+            lv[1][edge] = lv[0][edge]
             gid = b.add(kind, F64, dims, COEF[kind])
             for l in range(2):
                 b.upload(gid, l, lv[l])
