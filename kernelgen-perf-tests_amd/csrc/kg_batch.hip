@@ -59,9 +59,12 @@ constexpr int NTYPE = 2;        // KG_F64, KG_F32
 constexpr int NLPR = 3;         // plane-engine shapes: lanes per row 64, 32, 16
 constexpr int FLAT = NLPR;      // shape index of the flat (cache-served) body
 constexpr int PACK = NLPR + 1;  // shape index of packed whole rows
-constexpr int NSHAPE = NLPR + 2;
+constexpr int ROW2 = NLPR + 2;  // shape index of the direct row body (2D kinds only)
+constexpr int NSHAPE = NLPR + 3;
 constexpr int NVAR = NKIND * NTYPE * NSHAPE;
-constexpr int LPRS[NSHAPE] = {64, 32, 16, 0, -1};   // API codes: 0 flat, -1 packed rows
+// API codes: 0 flat, -1 packed rows, -2 direct rows
+constexpr int LPRS[NSHAPE] = {64, 32, 16, 0, -1, -2};
+constexpr bool is_2d(int kind) { return kind == KG_KIND_JACOBI2 || kind == KG_KIND_LIFE2; }
 constexpr int KCS[] = {8, 16, 32, 64};
 constexpr int NKC = sizeof(KCS) / sizeof(KCS[0]);
 
@@ -153,6 +156,14 @@ constexpr VarInfo var_info()
         using C = VarCfg<K, DT, 64>;
         return VarInfo{0, 0, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC, (int)sizeof(typename C::S),
                        C::H, kgp::Packed<C>::LDS_BYTES};
+    } else if constexpr (LI == ROW2) {
+        if constexpr (is_2d(K)) {
+            using C = VarCfg<K, DT, 64>;
+            return VarInfo{kgp::Rows2<C>::TXW, kgp::Rows2<C>::TY, B::LO, B::HI, B::ZLO, B::ZHI,
+                           C::VEC, (int)sizeof(typename C::S), C::H, 0};
+        } else {
+            return var_info<K, DT, 0>();   // never used: build_work maps it to shape 0
+        }
     } else {
         using C = VarCfg<K, DT, LPRS[LI]>;
         return VarInfo{C::TXW, C::TY, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC,
@@ -170,9 +181,10 @@ constexpr VarInfo var_info()
 #define KG_FLAT_LIST(X) \
     X(0, 0) X(0, 1) X(1, 0) X(1, 1) X(2, 0) X(2, 1) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
 
-#define KG_VAR_INFO5(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
-                            var_info<K, DT, FLAT>(), var_info<K, DT, PACK>(),
-constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO5)};
+#define KG_VAR_INFO6(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
+                            var_info<K, DT, FLAT>(), var_info<K, DT, PACK>(), var_info<K, DT, ROW2>(),
+constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO6)};
+#define KG_ROW2_LIST(X) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
 
 // Rows of a packed tile (halo rows included) for a grid of variant v with
 // row pitch sy; 0 when the grid's rows do not pack (see kgp::pack_rows).
@@ -324,6 +336,22 @@ __device__ __attribute__((noinline)) void run_pack(const GridDesc* __restrict__ 
                                                           uni(g.kc), op);
 }
 
+template <int K, int DT>
+__device__ __attribute__((noinline)) void run_rows2(const GridDesc* __restrict__ gp_, int bx_,
+                                                    int by_, int bz_, unsigned long long step_)
+{
+    static_assert(is_2d(K), "direct rows: single-plane kinds only");
+    const GridDesc& g = *uni(gp_);
+    const int bx = uni(bx_), by = uni(by_), bz = uni(bz_);
+    const unsigned long long step = uni(step_);
+    using C = VarCfg<K, DT, 64>;
+    using Op = typename Kind<K>::template Op<C>;
+    using B = typename Kind<K>::B;
+    const Op op = make_op<K, C>(g, step);
+    kgp::rows2_tile<C, B::LO, B::HI, Op>(uni(g.nx), uni(g.ny), uni(g.sy), uni(g.sz), bx, by,
+                                         B::ZLO + bz, op);
+}
+
 // Flat body: the workgroup sweeps the interior points of planes
 // ZLO + bz*kc .. +kc, one point per lane per round, x fastest; operands come
 // from global memory, where a small grid's planes stay cached (L1/L2).
@@ -402,6 +430,10 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
     case var_id(K, DT, PACK): run_pack<K, DT>(g, it.by, it.bz, step); break;
             KG_FLAT_LIST(KG_FLAT_CASE)
 #undef KG_FLAT_CASE
+#define KG_ROW2_CASE(K, DT) \
+    case var_id(K, DT, ROW2): run_rows2<K, DT>(g, it.bx, it.by, it.bz, step); break;
+            KG_ROW2_LIST(KG_ROW2_CASE)
+#undef KG_ROW2_CASE
         default: break;
         }
         __syncthreads();   // the next tile restages LDS
@@ -436,6 +468,15 @@ __global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
 void seq_flat_k(const GridDesc* __restrict__ gp, unsigned long long step)
 {
     run_flat<K, DT>(gp, (int)blockIdx.x, step);
+}
+
+template <int K, int DT>
+__global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
+void seq_rows2_k(const GridDesc* __restrict__ gp, int ntx, int nty, unsigned long long step)
+{
+    const unsigned b = blockIdx.x;
+    run_rows2<K, DT>(gp, (int)(b % (unsigned)ntx), (int)((b / (unsigned)ntx) % (unsigned)nty),
+                     (int)(b / ((unsigned)ntx * (unsigned)nty)), step);
 }
 
 template <int K, int DT>
@@ -599,13 +640,14 @@ int size_class(const kg_grid_spec& s)
     return ((s.kind * NTYPE + s.dtype) * 5 + bucket(s.nx)) * 5 + bucket(s.ny);
 }
 
+// Shape of a grid whose size class has not been tuned: the direct row body
+// for the 2D kinds; packed rows when a row fits in a wave (build_work falls
+// back to 64 lanes per row where rows do not pack), 64 lanes per row
+// otherwise.
 void default_shape(const kg_grid_spec& s, int& li, int& kc)
 {
-    const int vec = s.dtype == KG_F64 ? TypeOf<KG_F64>::VEC : TypeOf<KG_F32>::VEC;
-    li = 0;
-    if ((long long)s.nx * s.ny <= 1024) li = FLAT;
-    else if (s.nx <= 16 * vec) li = 2;
-    else if (s.nx <= 32 * vec) li = 1;
+    const int vec = vec_of(s.kind, s.dtype);
+    li = is_2d(s.kind) ? ROW2 : s.nx <= 64 * vec ? PACK : 0;
     kc = s.ns <= 24 ? 8 : s.ns <= 48 ? 16 : 32;
 }
 
@@ -646,6 +688,13 @@ void cut_tiles(const HostGrid& g, int gi, int li, int kc, std::vector<Item>& out
     if (nxi <= 0 || nyi <= 0 || nzi <= 0) return;
     if (li == FLAT) {
         for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++) out.push_back(Item{gi, 0, 0, bz});
+        return;
+    }
+    if (li == ROW2) {   // one plane per item
+        const int ntx = (int)((g.sy + v.txw - 1) / v.txw), nty = (nyi + v.ty - 1) / v.ty;
+        for (int bz = 0; bz < nzi; bz++)
+            for (int by = 0; by < nty; by++)
+                for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bz});
         return;
     }
     if (li == PACK) {
@@ -741,8 +790,10 @@ int build_work(kg_batch* b, int only_class = -1, int li_force = 0, int kc_force 
         } else {
             shape_of(b, g, li, kc);
         }
-        // grids whose rows do not pack take the 64-lane rows instead
+        // grids whose rows do not pack take the 64-lane rows instead; the
+        // direct row body is for the 2D kinds only
         if (li == PACK && !pack_rows_of(VARS[var_id(g.spec.kind, g.spec.dtype, PACK)], g.sy)) li = 0;
+        if (li == ROW2 && !is_2d(g.spec.kind)) li = 0;
         g.li = li;
         g.kc = kc;
         descs[gi] = make_desc(g, li, kc);
@@ -1072,8 +1123,8 @@ extern "C" int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int pl
         if (LPRS[q] == lanes_per_row) li = q;
     if (li < 0 || planes_per_tile < 1)
         return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: lanes per row must be 16, 32, "
-                                            "64, 0 (flat) or -1 (packed rows) and planes per "
-                                            "tile positive");
+                                            "64, 0 (flat), -1 (packed rows) or -2 (direct rows, "
+                                            "2D kinds) and planes per tile positive");
     b->tuned[size_class(b->grids[id].spec)] = {li, planes_per_tile};
     b->dirty = true;
     drop_graphs(b);
@@ -1163,6 +1214,21 @@ extern "C" int kg_batch_step_sequential(kg_batch* b, int nsteps)
         break;
                     KG_FLAT_LIST(KG_SEQ_FLAT)
 #undef KG_SEQ_FLAT
+                default: break;
+                }
+                continue;
+            }
+            if (g.li == ROW2) {
+                const GridDesc* gp = b->d_desc + gi;
+                const int ntx = (int)((g.sy + v.txw - 1) / v.txw), nty = (nyi + v.ty - 1) / v.ty;
+                switch (g.spec.kind * NTYPE + g.spec.dtype) {
+#define KG_SEQ_ROW2(K, DT)                                                                      \
+    case K * NTYPE + DT:                                                                        \
+        hipLaunchKernelGGL((seq_rows2_k<K, DT>), dim3((unsigned)(ntx * nty * nzi)), dim3(64 * BW), \
+                           0, b->ctx->stream, gp, ntx, nty, step);                              \
+        break;
+                    KG_ROW2_LIST(KG_SEQ_ROW2)
+#undef KG_SEQ_ROW2
                 default: break;
                 }
                 continue;

@@ -515,6 +515,83 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
     });
 }
 
+// ------------------------------------------------------------ direct rows
+// Single-plane stencils (the 2D kinds, applied to each xy-plane on its own)
+// without LDS: a lane owns VEC points of PY consecutive rows of plane k and
+// loads the rows it needs (three VEC-point pieces each, at x-VEC, x, x+VEC)
+// through the caches, all before the first use; the workgroup's W waves are
+// stacked in y.  A tile of the plane engine holds one plane here and would
+// pay its pipeline fill for every tile.  Requires H <= VEC and VEC-aligned,
+// padded rows (sy % VEC == 0).
+template <class C>
+struct Rows2 {
+    static constexpr int PY = 4;                         // output rows per lane
+    static constexpr int NR = PY + 2 * C::H;             // rows held per lane
+    static constexpr int TXW = 64 * C::VEC;              // tile width
+    static constexpr int TY = C::W * PY;                 // tile rows
+    static_assert(C::H <= C::VEC, "x-neighbours within the adjacent pieces");
+};
+
+template <class C, int R>
+struct RView {
+    using F = typename C::F;
+    const F (&win)[Rows2<C>::NR][3 * C::VEC];   // rows j0-H .., points x-VEC .. x+2VEC-1
+    bool wr;
+    __device__ __forceinline__ F at(int v, int dx, int dy, int) const
+    {
+        return win[R + C::H + dy][C::VEC + v + dx];
+    }
+    __device__ __forceinline__ F col(int v, int) const { return win[R + C::H][C::VEC + v]; }
+    __device__ __forceinline__ F side(int, int) const { return F(0); }
+    template <class S>
+    __device__ __forceinline__ void put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                        const bool (&xm)[C::VEC]) const;
+};
+
+// Tile (bx, by) of plane k: xy-interior [LO, n-HI).  No barriers: lanes with
+// nothing to do return at once.
+template <class C, int LO, int HI, class Op>
+__device__ __forceinline__ void rows2_tile(int nx, int ny, long long sy, long long sz, int bx,
+                                           int by, int k, const Op& op)
+{
+    using R2 = Rows2<C>;
+    using S = typename C::S;
+    using F = typename C::F;
+    using T = typename C::T;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int x = bx * R2::TXW + lane * C::VEC;
+    const int j0 = LO + (by * C::W + w) * R2::PY;
+    if (x >= sy || j0 >= ny - HI) return;
+    // pieces beside the row ends are clamped into the row: they then feed
+    // only boundary points, which take the centre value
+    const int xl = x >= C::VEC ? x - C::VEC : x;
+    const int xr = x + C::VEC < sy ? x + C::VEC : x;
+    const gmem<S>* base = as_global(op.src) + (long long)k * sz;
+    F win[R2::NR][3 * C::VEC];
+#pragma unroll
+    for (int q = 0; q < R2::NR; q++) {
+        const gmem<S>* row = base + (long long)min(j0 - C::H + q, ny - 1) * sy;
+        F a[C::VEC], b[C::VEC], c[C::VEC];
+        vload<T, C::VEC>(row + xl, a);
+        vload<T, C::VEC>(row + x, b);
+        vload<T, C::VEC>(row + xr, c);
+#pragma unroll
+        for (int v = 0; v < C::VEC; v++) {
+            win[q][v] = a[v];
+            win[q][C::VEC + v] = b[v];
+            win[q][2 * C::VEC + v] = c[v];
+        }
+    }
+    bool xmask[C::VEC];
+#pragma unroll
+    for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
+    kg_static_for<R2::PY>([&](auto rr) {
+        constexpr int R = decltype(rr)::value;
+        const int j = j0 + R;
+        if (j < ny - HI) op.run(RView<C, R>{win, true}, (long long)k * sz + (long long)j * sy, x, xmask);
+    });
+}
+
 // Single-grid launch in the dense layout: one workgroup per tile, tiles
 // enumerated x fastest, then y, then z-chunk, through the XCD swizzle when
 // SWZ (halo rows then hit in the XCD's L2; without it the chip streams one
@@ -589,6 +666,13 @@ template <class C, int P>
 template <class S>
 __device__ __forceinline__ void View<C, P>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
                                                 const bool (&xm)[C::VEC]) const
+{
+    put_row<C>(*this, dst, ob, x, r, xm);
+}
+template <class C, int R>
+template <class S>
+__device__ __forceinline__ void RView<C, R>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                                 const bool (&xm)[C::VEC]) const
 {
     put_row<C>(*this, dst, ob, x, r, xm);
 }

@@ -89,12 +89,15 @@ def test_serve_jacobi_stops_at_tolerance(gpu_ctx, kgmod):
     assert np.max(np.abs(nxt - w)) < 1e-8
 
 
+@pytest.mark.parametrize("shape", [None, 64, -1])
 @pytest.mark.parametrize("kind", ["jacobi2", "life2"])
-def test_batch_2d_kinds_equal_dropin_per_plane(gpu_ctx, kgmod, kind):
+def test_batch_2d_kinds_equal_dropin_per_plane(gpu_ctx, kgmod, kind, shape):
+    """Default shape (the direct row body), the plane engine with 64 lanes per
+    row, and packed rows: each the 2D entry point applied to every plane."""
     rng = np.random.default_rng(3)
     b = kgmod.Batch(gpu_ctx)
     try:
-        for dims in [(37, 21, 1), (64, 40, 3), (5, 4, 2), (130, 17, 1)]:
+        for dims in [(37, 21, 1), (64, 40, 3), (5, 4, 2), (130, 17, 1), (300, 70, 2), (1030, 9, 1)]:
             nx, ny, ns = dims
             lv = [rng.uniform(-1, 1, (ns, ny, nx)) for _ in range(2)]
             if kind == "life2":
@@ -104,6 +107,8 @@ def test_batch_2d_kinds_equal_dropin_per_plane(gpu_ctx, kgmod, kind):
             edge[:, 1:ny - 1, 1:nx - 1] = False
             lv[1][edge] = lv[0][edge]
             gid = b.add(kind, F64, dims, COEF[kind])
+            if shape is not None:
+                b.set_shape(gid, shape, 4)
             for l in range(2):
                 b.upload(gid, l, lv[l])
             b.step(3, graph=True)
@@ -123,3 +128,32 @@ def test_batch_2d_kinds_equal_dropin_per_plane(gpu_ctx, kgmod, kind):
             b.remove(gid)
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("kind", ["jacobi2", "life2"])
+def test_batch_2d_kinds_fp32_same_in_every_shape(gpu_ctx, kgmod, kind):
+    """FP32 2D kinds have no single-grid entry point: the direct row body, the
+    plane engine and packed rows must agree bit for bit (same op body)."""
+    rng = np.random.default_rng(4)
+    dims = (131, 45, 2)
+    nx, ny, ns = dims
+    lv = [rng.uniform(-1, 1, (ns, ny, nx)).astype(np.float32) for _ in range(2)]
+    if kind == "life2":
+        lv = [(a > 0.2).astype(np.float32) for a in lv]
+    edge = np.ones((ns, ny, nx), bool)
+    edge[:, 1:ny - 1, 1:nx - 1] = False
+    lv[1][edge] = lv[0][edge]
+    outs = []
+    for shape in (-2, 64, -1, 16):
+        b = kgmod.Batch(gpu_ctx)
+        try:
+            gid = b.add(kind, F32, dims, COEF[kind])
+            b.set_shape(gid, shape, 8)
+            for l in range(2):
+                b.upload(gid, l, lv[l])
+            b.step(3, graph=False)
+            outs.append(b.download(gid))
+        finally:
+            b.close()
+    for o in outs[1:]:
+        assert o.tobytes() == outs[0].tobytes(), kind
