@@ -70,6 +70,14 @@ int kg_check_ctx(kg_ctx* ctx);
 int kg_check_dims3(int nx, int ny, int ns);
 int kg_check_dims2(int nx, int ny);
 int kg_check_plane_bytes(long long points, int elem_bytes);
+// True when every pointer is 16-byte aligned: the kernels' 16-byte lanes
+// (two doubles, four floats) are taken only then, 8-byte or narrower ones
+// otherwise.
+template <class... P>
+inline bool kg_aligned16(const P*... p)
+{
+    return !((0u | ... | (unsigned)((uintptr_t)p & 15u)));
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; each XCD
 // has its own L2.  kg_xcd_swizzle maps the hardware workgroup id to a logical

@@ -147,10 +147,11 @@ template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 2, 0, 64>;
 template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 2, 0, 32>;
 template <int V> using WhCfg = kgr::Cfg<V, 8 / V, 1, 5, 0, 0, 1, 1, 2, 3, 8>;
 
-// Launch Op on the row engine; 16-byte lanes (VEC=2) need an even nx.
+// Launch Op on the row engine; 16-byte lanes (VEC=2) need an even nx and
+// 16-byte aligned fields (al).
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int VEC_EVEN,
           class Fill>
-static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, Fill fill)
+static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
     if (int rc = kg_check_dims2(nx, ny)) return rc;
@@ -169,7 +170,7 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, Fill fill)
                                g.ntx, op);
         return kg_launch_end(ctx);
     };
-    if (VEC_EVEN == 2 && nx % 2 == 0) return go(std::integral_constant<int, VEC_EVEN>{});
+    if (VEC_EVEN == 2 && nx % 2 == 0 && al) return go(std::integral_constant<int, VEC_EVEN>{});
     return go(std::integral_constant<int, 1>{});
 }
 
@@ -199,8 +200,8 @@ template <class DC, template <int> class CfgT, template <class> class OpT, int L
 static int launch_direct(kg_ctx* ctx, const char* name, int nx, int ny, const void* in,
                          const void* out, Fill fill)
 {
-    if (nx % 2 != 0 || (((uintptr_t)in | (uintptr_t)out) & 15))
-        return launch_row<CfgT, OpT, LO, HI, 2>(ctx, name, nx, ny, fill);
+    const bool al = kg_aligned16(in, out);
+    if (nx % 2 != 0 || !al) return launch_row<CfgT, OpT, LO, HI, 2>(ctx, name, nx, ny, al, fill);
     if (int rc = kg_check_ctx(ctx)) return rc;
     if (int rc = kg_check_dims2(nx, ny)) return rc;
     using Op = OpT<DC>;
@@ -275,7 +276,9 @@ extern "C" int kg_whispering(kg_ctx* ctx, int nx, int ny, double mu, double epsi
     KG_REQUIRE(e_in != e_out && h0 != h1 && u_em0 != u_em1,
                "kg_whispering: e, h and u_em need distinct in/out buffers");
     const long long n = (long long)nx * ny;
-    return launch_row<WhCfg, WhisperingOp, 1, 1, 2>(ctx, "whispering", nx, ny, [&](auto& op) {
+    const bool al = kg_aligned16(e_in, e_out, h0, h1, u_em0, u_em1, ca, cb, da, db) &&
+                    kg_aligned16(e_in + n, e_out + n);
+    return launch_row<WhCfg, WhisperingOp, 1, 1, 2>(ctx, "whispering", nx, ny, al, [&](auto& op) {
         op.src[0] = h0;
         op.src[1] = ca;
         op.src[2] = cb;

@@ -328,11 +328,12 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #endif
 template <int V> using GsrbCfg = kgp::Cfg<1, KG_GSRB_W, 2, 1, 1, 2, 2, KG_GSRB_D, 0, KG_GSRB_KC>;
 
-// Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx.
+// Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx and
+// 16-byte aligned fields (al).
 // SWZ: XCD-contiguous tile order (measured per kernel at 512x256x256).
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, bool SWZ = true,
           class Fill>
-static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, Fill fill)
+static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
     if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
@@ -352,7 +353,7 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, F
                                nx, ny, ns, g.ntx, g.nty, op);
         return kg_launch_end(ctx);
     };
-    if (nx % 2 == 0) return go(std::integral_constant<int, 2>{});
+    if (nx % 2 == 0 && al) return go(std::integral_constant<int, 2>{});
     return go(std::integral_constant<int, 1>{});
 }
 
@@ -378,7 +379,8 @@ extern "C" int kg_laplacian(kg_ctx* ctx, int nx, int ny, int ns, double alpha, d
 {
     KG_REQUIRE(w0 && w1, "kg_laplacian: null buffer");
     KG_REQUIRE(w0 != w1, "kg_laplacian: w0 and w1 must be distinct (out-of-place update)");
-    return launch_plane<LapCfg, LaplacianOp, 1, 1>(ctx, "laplacian", nx, ny, ns, [&](auto& op) {
+    return launch_plane<LapCfg, LaplacianOp, 1, 1>(ctx, "laplacian", nx, ny, ns,
+                                                   kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.alpha = alpha;
@@ -412,7 +414,8 @@ extern "C" int kg_gradient(kg_ctx* ctx, int nx, int ny, int ns, double alpha, do
     KG_REQUIRE(u && ux && uy && uz, "kg_gradient: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz && ux != uy && ux != uz && uy != uz,
                "kg_gradient: buffers must be distinct");
-    return launch_plane<LapCfg, GradientOp, 1, 1, false>(ctx, "gradient", nx, ny, ns, [&](auto& op) {
+    return launch_plane<LapCfg, GradientOp, 1, 1, false>(ctx, "gradient", nx, ny, ns,
+                                                         kg_aligned16(u, ux, uy, uz), [&](auto& op) {
         op.src = u;
         op.ux = ux;
         op.uy = uy;
@@ -428,7 +431,8 @@ extern "C" int kg_wave13pt(kg_ctx* ctx, int nx, int ny, int ns, double m0, doubl
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt: w2 aliases an input");
-    return launch_plane<WaveCfg, Wave13ptOp, 2, 2>(ctx, "wave13pt", nx, ny, ns, [&](auto& op) {
+    return launch_plane<WaveCfg, Wave13ptOp, 2, 2>(ctx, "wave13pt", nx, ny, ns,
+                                                   kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
         op.dst = w2;
@@ -467,7 +471,8 @@ extern "C" int kg_lapgsrb(kg_ctx* ctx, int nx, int ny, int ns, double c0, double
 {
     KG_REQUIRE(w0 && w1, "kg_lapgsrb: null buffer");
     KG_REQUIRE(w0 != w1, "kg_lapgsrb: w0 and w1 must be distinct");
-    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, false>(ctx, "lapgsrb", nx, ny, ns, [&](auto& op) {
+    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, false>(ctx, "lapgsrb", nx, ny, ns,
+                                                         kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.c0 = c0;

@@ -26,7 +26,7 @@ using namespace kgops;
 // Single-grid launch of op Op in storage type T: 16-byte lanes (VEC = VMAX)
 // when nx is a multiple of VMAX, else one point per lane.
 template <template <class> class OpT, class T, int VMAX, int W, int D, int KC, class Fill>
-static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, Fill fill)
+static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
     if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
@@ -48,7 +48,7 @@ static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, F
                                nx, ny, ns, g.ntx, g.nty, op);
         return kg_launch_end(ctx);
     };
-    if (nx % VMAX == 0) return go(std::integral_constant<int, VMAX>{});
+    if (nx % VMAX == 0 && al) return go(std::integral_constant<int, VMAX>{});
     return go(std::integral_constant<int, 1>{});
 }
 
@@ -228,7 +228,7 @@ extern "C" int kg_laplacian_f32(kg_ctx* ctx, int nx, int ny, int ns, float alpha
     KG_REQUIRE(w0 && w1, "kg_laplacian_f32: null buffer");
     KG_REQUIRE((const void*)w0 != (const void*)w1, "kg_laplacian_f32: w0 and w1 must be distinct");
     return launch_typed<LaplacianOp, kgp::F32, 4, KG_LF32_W, KG_LF32_D, KG_LF32_KC>(ctx, "laplacian_f32", nx, ny, ns,
-                                                            [&](auto& op) {
+                                                            kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.alpha = alpha;
@@ -242,7 +242,7 @@ extern "C" int kg_laplacian_bf16(kg_ctx* ctx, int nx, int ny, int ns, float alph
     KG_REQUIRE(w0 && w1, "kg_laplacian_bf16: null buffer");
     KG_REQUIRE((const void*)w0 != (const void*)w1, "kg_laplacian_bf16: w0 and w1 must be distinct");
     return launch_typed<LaplacianOp, kgp::BF16, 8, 8, 2, 16>(ctx, "laplacian_bf16", nx, ny, ns,
-                                                             [&](auto& op) {
+                                                             kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.alpha = alpha;
@@ -256,7 +256,7 @@ extern "C" int kg_wave13pt_f32(kg_ctx* ctx, int nx, int ny, int ns, float m0, fl
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt_f32: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_f32: w2 aliases an input");
     return launch_typed<Wave13ptOp, kgp::F32, 4, KG_WF32_W, KG_WF32_D, KG_WF32_KC>(ctx, "wave13pt_f32", nx, ny, ns,
-                                                            [&](auto& op) {
+                                                            kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
         op.dst = w2;
@@ -274,7 +274,7 @@ extern "C" int kg_wave13pt_bf16(kg_ctx* ctx, int nx, int ny, int ns, float m0, f
     // 8-byte lanes: with 16-byte lanes (8 points) the 1024-thread tile needs
     // more than the 128 registers a lane may hold at that size and spills.
     return launch_typed<Wave13ptOp, kgp::BF16, 4, 16, 2, 32>(ctx, "wave13pt_bf16", nx, ny, ns,
-                                                             [&](auto& op) {
+                                                             kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
         op.dst = w2;
@@ -292,7 +292,7 @@ extern "C" int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0,
     KG_REQUIRE(w0 != w1, "kg_gaussblur3_f64: w0 and w1 must be distinct");
     const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
     return launch_typed<Gaussblur3Op, kgp::F64, 2, 8, 2, 16>(ctx, "gaussblur3_f64", nx, ny, ns,
-                                                             [&](auto& op) {
+                                                             kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.f = f;
@@ -312,7 +312,7 @@ extern "C" int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, 
     KG_REQUIRE(w0 != w1, "kg_gaussblur3_f32: w0 and w1 must be distinct");
     const float f = 1.f / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);
     return launch_typed<Gaussblur3Op, kgp::F32, 4, 8, 2, 16>(ctx, "gaussblur3_f32", nx, ny, ns,
-                                                             [&](auto& op) {
+                                                             kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
         op.f = f;

@@ -263,6 +263,17 @@ class RefBackend(_HostBackend):
         super().__init__(path, "", True)
 
 
+class _Shifted:
+    """A device array seen `offset` doubles past its start."""
+
+    def __init__(self, arr, offset, shape):
+        self.arr, self.offset, self.shape = arr, offset, shape
+        self.ptr = arr.ptr + 8 * offset
+
+    def download(self):
+        return self.arr.download()[self.offset:].reshape(self.shape)
+
+
 class GpuBackend:
     """Runs a spec on the device through the C ABI; buffers stay resident
     across steps, as in the reference's cuda target."""
@@ -270,17 +281,27 @@ class GpuBackend:
     def __init__(self, ctx):
         self.ctx = ctx
 
-    def run(self, spec: Spec, dims, coefs: dict, arrays: dict, nt: int) -> dict:
+    def run(self, spec: Spec, dims, coefs: dict, arrays: dict, nt: int, offset: int = 0) -> dict:
+        """offset > 0 places every array that many doubles past the start of
+        its allocation (8-byte but not 16-byte aligned for odd offsets)."""
         ctx = self.ctx
-        st = {r: ctx.array(np.ascontiguousarray(a, dtype=np.float64)) for r, a in arrays.items()}
+        src = dict(arrays)
         if spec.name == "whispering":
-            st["e_next"] = ctx.array(np.ascontiguousarray(arrays["e"], dtype=np.float64))
+            src["e_next"] = arrays["e"]
+        allocs, st = {}, {}
+        for r, a in src.items():
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if offset:
+                allocs[r] = ctx.array(np.concatenate([np.zeros(offset), a.ravel()]))
+                st[r] = _Shifted(allocs[r], offset, a.shape)
+            else:
+                allocs[r] = st[r] = ctx.array(a)
         for _ in range(nt):
             self.step(spec, dims, coefs, st)
             rotate(spec, st)
         ctx.sync()
         out = {r: st[r].download() for r in arrays}
-        for v in st.values():
+        for v in allocs.values():
             v.free()
         return out
 
@@ -290,5 +311,6 @@ class GpuBackend:
             arrs = [st["e"], st["e_next"]] + [st[r] for r in spec.arrays[1:]]
         else:
             arrs = [st[r] for r in spec.arrays]
-        self.ctx.call(spec.name, *[int(d) for d in dims], *cargs, *arrs)
+        self.ctx.call(spec.name, *[int(d) for d in dims], *cargs,
+                      *[a.ptr if isinstance(a, _Shifted) else a for a in arrs])
 

@@ -51,6 +51,20 @@ def test_gpu_matches_oracle_ragged(gpu_ctx, oracle, name):
                 assert_bit_equal(got[role], want[role], f"{name} {dims} nt={nt} {role}")
 
 
+@pytest.mark.parametrize("name", K.STENCILS)
+def test_gpu_matches_oracle_unaligned(gpu_ctx, oracle, name):
+    """Fields 8 bytes past a 16-byte boundary (a caller's sub-array): the
+    entry points then take their 8-byte-lane paths, with the same results."""
+    spec = K.SPECS[name]
+    sizes = [(66, 9, 35), (512, 8, 21)] if spec.dim == 3 else [(256, 37), (512, 100)]
+    for si, dims in enumerate(sizes):
+        coefs, arrays = K.random_case(spec, dims, 700 + si)
+        want = oracle.run(spec, dims, coefs, arrays, 2)
+        got = K.GpuBackend(gpu_ctx).run(spec, dims, coefs, arrays, 2, offset=1)
+        for role in want:
+            assert_bit_equal(got[role], want[role], f"{name} {dims} unaligned {role}")
+
+
 FULL3 = (512, 256, 256)
 FULL2 = (512, 65536)
 
