@@ -287,6 +287,8 @@ class Batch:
     """Many independent 3D grids stepped together (kg_batch_* of the C ABI).
 
     Host arrays are dense (ns, ny, nx) in the grid's dtype (float64/float32).
+    A grid's levels share one boundary: the cells outside the kind's interior
+    box must hold the same values in every uploaded level (fill() does so).
     """
 
     def __init__(self, ctx: Context):
@@ -378,6 +380,9 @@ class Batch:
         return a.value, b.value
 
     def set_shape(self, gid: int, lanes_per_row: int, planes_per_tile: int) -> None:
+        """Forces the tile shape of grid gid's size class: 64/32/16 lanes per
+        row, 0 the flat body, -1 packed rows, -2 the direct row body (2D
+        kinds)."""
         _check(self.lib, self.lib.kg_batch_set_shape(self.ptr, gid, lanes_per_row,
                                                      planes_per_tile), "kg_batch_set_shape")
 
