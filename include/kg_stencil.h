@@ -289,12 +289,13 @@ int kg_batch_info(kg_batch* b, double* points_per_step, double* bytes_per_step,
 /* Autotuner: for every size class present (kind, type, nx and ny bucket),
  * times each tile shape -- the plane engine with 16/32/64 lanes per row, with
  * whole rows packed back to back across the workgroup (lanes_per_row -1, for
- * rows narrower than a wave), the direct row body of the 2D kinds
- * (lanes_per_row -2: rows loaded through the caches, one plane per tile), or
+ * rows narrower than a wave), the direct row body of the planar kinds
+ * (gaussblur3, jacobi2, life2; lanes_per_row -2: rows loaded through the
+ * caches, one plane per tile), or
  * the flat body (lanes_per_row 0: points swept from cache, for small grids)
  * -- with 8/16/32/64 planes per tile, over reps launches on the batch's own
  * grids, and keeps the fastest.  Untuned classes take the direct row body
- * (2D kinds), packed rows (rows up to 64 lanes) or 64 lanes per row.  Grid
+ * (jacobi2, life2), packed rows (rows up to 64 lanes) or 64 lanes per row.  Grid
  * contents are preserved. */
 int kg_batch_tune(kg_batch* b, int reps, int* classes_tuned);
 int kg_batch_grid_shape(kg_batch* b, int id, int* lanes_per_row, int* planes_per_tile);

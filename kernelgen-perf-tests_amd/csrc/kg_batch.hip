@@ -59,12 +59,17 @@ constexpr int NTYPE = 2;        // KG_F64, KG_F32
 constexpr int NLPR = 3;         // plane-engine shapes: lanes per row 64, 32, 16
 constexpr int FLAT = NLPR;      // shape index of the flat (cache-served) body
 constexpr int PACK = NLPR + 1;  // shape index of packed whole rows
-constexpr int ROW2 = NLPR + 2;  // shape index of the direct row body (2D kinds only)
+constexpr int ROW2 = NLPR + 2;  // shape index of the direct row body (planar kinds only)
 constexpr int NSHAPE = NLPR + 3;
 constexpr int NVAR = NKIND * NTYPE * NSHAPE;
 // API codes: 0 flat, -1 packed rows, -2 direct rows
 constexpr int LPRS[NSHAPE] = {64, 32, 16, 0, -1, -2};
-constexpr bool is_2d(int kind) { return kind == KG_KIND_JACOBI2 || kind == KG_KIND_LIFE2; }
+// Kinds whose stencil reads only the plane it writes (gaussblur3 and the 2D
+// kinds, applied to every xy-plane): they can run the direct row body.
+constexpr bool is_planar(int kind)
+{
+    return kind == KG_KIND_GAUSSBLUR3 || kind == KG_KIND_JACOBI2 || kind == KG_KIND_LIFE2;
+}
 constexpr int KCS[] = {8, 16, 32, 64};
 constexpr int NKC = sizeof(KCS) / sizeof(KCS[0]);
 
@@ -118,9 +123,15 @@ constexpr int depth_of(int kind, int dt)
 
 // Points per lane: 16-byte lanes, except wave13pt (13 neighbours, 3 levels),
 // whose 16-byte FP64/FP32 bodies do not fit 128 VGPRs with a useful prefetch.
+#ifndef KG_BATCH_WVEC64
+#define KG_BATCH_WVEC64 1
+#endif
+#ifndef KG_BATCH_WVEC32
+#define KG_BATCH_WVEC32 2
+#endif
 constexpr int vec_of(int kind, int dt)
 {
-    if (kind == KG_KIND_WAVE13PT) return dt == KG_F64 ? 1 : 2;
+    if (kind == KG_KIND_WAVE13PT) return dt == KG_F64 ? KG_BATCH_WVEC64 : KG_BATCH_WVEC32;
     return dt == KG_F64 ? TypeOf<KG_F64>::VEC : TypeOf<KG_F32>::VEC;
 }
 
@@ -157,7 +168,7 @@ constexpr VarInfo var_info()
         return VarInfo{0, 0, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC, (int)sizeof(typename C::S),
                        C::H, kgp::Packed<C>::LDS_BYTES};
     } else if constexpr (LI == ROW2) {
-        if constexpr (is_2d(K)) {
+        if constexpr (is_planar(K)) {
             using C = VarCfg<K, DT, 64>;
             return VarInfo{kgp::Rows2<C>::TXW, kgp::Rows2<C>::TY, B::LO, B::HI, B::ZLO, B::ZHI,
                            C::VEC, (int)sizeof(typename C::S), C::H, 0};
@@ -184,7 +195,7 @@ constexpr VarInfo var_info()
 #define KG_VAR_INFO6(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
                             var_info<K, DT, FLAT>(), var_info<K, DT, PACK>(), var_info<K, DT, ROW2>(),
 constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO6)};
-#define KG_ROW2_LIST(X) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
+#define KG_ROW2_LIST(X) X(2, 0) X(2, 1) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
 
 // Rows of a packed tile (halo rows included) for a grid of variant v with
 // row pitch sy; 0 when the grid's rows do not pack (see kgp::pack_rows).
@@ -340,7 +351,7 @@ template <int K, int DT>
 __device__ __attribute__((noinline)) void run_rows2(const GridDesc* __restrict__ gp_, int bx_,
                                                     int by_, int bz_, unsigned long long step_)
 {
-    static_assert(is_2d(K), "direct rows: single-plane kinds only");
+    static_assert(is_planar(K), "direct rows: kinds that read only their own plane");
     const GridDesc& g = *uni(gp_);
     const int bx = uni(bx_), by = uni(by_), bz = uni(bz_);
     const unsigned long long step = uni(step_);
@@ -641,13 +652,15 @@ int size_class(const kg_grid_spec& s)
 }
 
 // Shape of a grid whose size class has not been tuned: the direct row body
-// for the 2D kinds; packed rows when a row fits in a wave (build_work falls
-// back to 64 lanes per row where rows do not pack), 64 lanes per row
-// otherwise.
+// for the 2D kinds (for gaussblur3 it measured slower than the plane engine:
+// serving capacity 196 against 210 jobs/s); packed rows when a row fits in a
+// wave (build_work falls back to 64 lanes per row where rows do not pack),
+// 64 lanes per row otherwise.
 void default_shape(const kg_grid_spec& s, int& li, int& kc)
 {
     const int vec = vec_of(s.kind, s.dtype);
-    li = is_2d(s.kind) ? ROW2 : s.nx <= 64 * vec ? PACK : 0;
+    const bool two_d = s.kind == KG_KIND_JACOBI2 || s.kind == KG_KIND_LIFE2;
+    li = two_d ? ROW2 : s.nx <= 64 * vec ? PACK : 0;
     kc = s.ns <= 24 ? 8 : s.ns <= 48 ? 16 : 32;
 }
 
@@ -791,9 +804,9 @@ int build_work(kg_batch* b, int only_class = -1, int li_force = 0, int kc_force 
             shape_of(b, g, li, kc);
         }
         // grids whose rows do not pack take the 64-lane rows instead; the
-        // direct row body is for the 2D kinds only
+        // direct row body is for the planar kinds only
         if (li == PACK && !pack_rows_of(VARS[var_id(g.spec.kind, g.spec.dtype, PACK)], g.sy)) li = 0;
-        if (li == ROW2 && !is_2d(g.spec.kind)) li = 0;
+        if (li == ROW2 && !is_planar(g.spec.kind)) li = 0;
         g.li = li;
         g.kc = kc;
         descs[gi] = make_desc(g, li, kc);
@@ -1124,7 +1137,7 @@ extern "C" int kg_batch_set_shape(kg_batch* b, int id, int lanes_per_row, int pl
     if (li < 0 || planes_per_tile < 1)
         return kg_set_error(KG_ERR_INVALID, "kg_batch_set_shape: lanes per row must be 16, 32, "
                                             "64, 0 (flat), -1 (packed rows) or -2 (direct rows, "
-                                            "2D kinds) and planes per tile positive");
+                                            "planar kinds) and planes per tile positive");
     b->tuned[size_class(b->grids[id].spec)] = {li, planes_per_tile};
     b->dirty = true;
     drop_graphs(b);

@@ -525,7 +525,7 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
 // padded rows (sy % VEC == 0).
 template <class C>
 struct Rows2 {
-    static constexpr int PY = 4;                         // output rows per lane
+    static constexpr int PY = C::H == 1 ? 4 : 2;         // output rows per lane (fits 120 VGPRs)
     static constexpr int NR = PY + 2 * C::H;             // rows held per lane
     static constexpr int TXW = 64 * C::VEC;              // tile width
     static constexpr int TY = C::W * PY;                 // tile rows

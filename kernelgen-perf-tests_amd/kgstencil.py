@@ -381,8 +381,8 @@ class Batch:
 
     def set_shape(self, gid: int, lanes_per_row: int, planes_per_tile: int) -> None:
         """Forces the tile shape of grid gid's size class: 64/32/16 lanes per
-        row, 0 the flat body, -1 packed rows, -2 the direct row body (2D
-        kinds)."""
+        row, 0 the flat body, -1 packed rows, -2 the direct row body (planar
+        kinds: gaussblur3, jacobi2, life2)."""
         _check(self.lib, self.lib.kg_batch_set_shape(self.ptr, gid, lanes_per_row,
                                                      planes_per_tile), "kg_batch_set_shape")
 

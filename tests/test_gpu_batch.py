@@ -181,18 +181,20 @@ def test_batch_shapes_and_tuning_do_not_change_results(gpu_ctx, batch):
     _check_all(gpu_ctx, batch, specs, ids, init, 18, "after tuning")
 
 
-@pytest.mark.parametrize("kc", [3, 16])
-def test_batch_packed_rows_match_oracle(gpu_ctx, batch, kc):
-    """Every grid in packed-row tiles (whole rows back to back across the
-    workgroup): rows of 3..69 points, FP64 and FP32, all kinds."""
+@pytest.mark.parametrize("shape,kc", [(-1, 3), (-1, 16), (-2, 8)])
+def test_batch_packed_and_direct_rows_match_single_grid(gpu_ctx, batch, shape, kc):
+    """Every grid in packed-row tiles (-1: whole rows back to back across the
+    workgroup) or, for gaussblur3, the direct row body (-2; the other kinds
+    then take 64 lanes per row): rows of 3..69 points, FP64 and FP32."""
     rng = np.random.default_rng(40 + kc)
     specs = _mixed_specs(rng, 24) + [("wave13pt", F64, (5, 70, 6)), ("gaussblur3", F32, (69, 5, 3)),
-                                     ("laplacian", F32, (3, 3, 3))]
+                                     ("laplacian", F32, (3, 3, 3)), ("gaussblur3", F64, (300, 41, 2)),
+                                     ("gaussblur3", F32, (515, 9, 3))]
     ids, init = _fill(batch, specs, rng)
     for g in ids:
-        batch.set_shape(g, -1, kc)
+        batch.set_shape(g, shape, kc)
     batch.step(3, graph=True)
-    _check_all(gpu_ctx, batch, specs, ids, init, 3, f"packed rows kc={kc}")
+    _check_all(gpu_ctx, batch, specs, ids, init, 3, f"shape {shape} kc={kc}")
 
 
 def test_batch_grids_join_and_leave(gpu_ctx, batch):
