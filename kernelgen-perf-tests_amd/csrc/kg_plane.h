@@ -523,19 +523,19 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
 // stacked in y.  A tile of the plane engine holds one plane here and would
 // pay its pipeline fill for every tile.  Requires H <= VEC and VEC-aligned,
 // padded rows (sy % VEC == 0).
-template <class C>
+template <class C, int PY_ = (C::H == 1 ? 4 : 2)>   // default fits 120 VGPRs
 struct Rows2 {
-    static constexpr int PY = C::H == 1 ? 4 : 2;         // output rows per lane (fits 120 VGPRs)
+    static constexpr int PY = PY_;                       // output rows per lane
     static constexpr int NR = PY + 2 * C::H;             // rows held per lane
     static constexpr int TXW = 64 * C::VEC;              // tile width
     static constexpr int TY = C::W * PY;                 // tile rows
     static_assert(C::H <= C::VEC, "x-neighbours within the adjacent pieces");
 };
 
-template <class C, int R>
+template <class C, int R, int NR>
 struct RView {
     using F = typename C::F;
-    const F (&win)[Rows2<C>::NR][3 * C::VEC];   // rows j0-H .., points x-VEC .. x+2VEC-1
+    const F (&win)[NR][3 * C::VEC];   // rows j0-H .., points x-VEC .. x+2VEC-1
     bool wr;
     __device__ __forceinline__ F at(int v, int dx, int dy, int) const
     {
@@ -550,11 +550,11 @@ struct RView {
 
 // Tile (bx, by) of plane k: xy-interior [LO, n-HI).  No barriers: lanes with
 // nothing to do return at once.
-template <class C, int LO, int HI, class Op>
+template <class C, int LO, int HI, class Op, int PY = Rows2<C>::PY>
 __device__ __forceinline__ void rows2_tile(int nx, int ny, long long sy, long long sz, int bx,
                                            int by, int k, const Op& op)
 {
-    using R2 = Rows2<C>;
+    using R2 = Rows2<C, PY>;
     using S = typename C::S;
     using F = typename C::F;
     using T = typename C::T;
@@ -588,8 +588,22 @@ __device__ __forceinline__ void rows2_tile(int nx, int ny, long long sy, long lo
     kg_static_for<R2::PY>([&](auto rr) {
         constexpr int R = decltype(rr)::value;
         const int j = j0 + R;
-        if (j < ny - HI) op.run(RView<C, R>{win, true}, (long long)k * sz + (long long)j * sy, x, xmask);
+        if (j < ny - HI)
+            op.run(RView<C, R, R2::NR>{win, true}, (long long)k * sz + (long long)j * sy, x, xmask);
     });
+}
+
+// Single-grid launch of the direct row body over every plane of a dense
+// grid: tiles enumerated x fastest, then y, then plane, in hardware order.
+template <class C, int LO, int HI, class Op, int PY>
+__global__ __launch_bounds__(64 * C::W)
+void rows2_kernel(int nx, int ny, int ntx, int nty, Op op)
+{
+    const unsigned b = blockIdx.x;
+    const int bx = (int)(b % (unsigned)ntx);
+    const int by = (int)((b / (unsigned)ntx) % (unsigned)nty);
+    const int k = (int)(b / ((unsigned)ntx * (unsigned)nty));
+    rows2_tile<C, LO, HI, Op, PY>(nx, ny, nx, (long long)nx * ny, bx, by, k, op);
 }
 
 // Single-grid launch in the dense layout: one workgroup per tile, tiles
@@ -669,10 +683,10 @@ __device__ __forceinline__ void View<C, P>::put(S* dst, long long ob, int x, F (
 {
     put_row<C>(*this, dst, ob, x, r, xm);
 }
-template <class C, int R>
+template <class C, int R, int NR>
 template <class S>
-__device__ __forceinline__ void RView<C, R>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
-                                                 const bool (&xm)[C::VEC]) const
+__device__ __forceinline__ void RView<C, R, NR>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                                     const bool (&xm)[C::VEC]) const
 {
     put_row<C>(*this, dst, ob, x, r, xm);
 }

@@ -87,13 +87,14 @@ def test_typed_matches_oracle(gpu_ctx, oracle, which, dtype):
 
 
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("wave13pt", "f32"),
-                                         ("laplacian", "bf16")])
+                                         ("laplacian", "bf16"), ("gaussblur3", "f32"),
+                                         ("gaussblur3", "f64")])
 def test_typed_full_size_one_step(gpu_ctx, oracle, which, dtype):
     """512x256x256, one step, bit-exact against the oracle."""
     dims = (512, 256, 256)
     fn = f"{which}_{dtype}"
     ins, out, coefs, rot = _case(which, dtype, dims, 7)
-    want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 1, rot, f32)
+    want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 1, rot, f64 if dtype == "f64" else f32)
     got = _run_gpu(gpu_ctx, fn, dims, coefs, ins, out, 1, rot)
     for g, w in zip(got, want):
         assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
