@@ -16,6 +16,13 @@
 // bilinear prolongation, damped-Jacobi smoothing, rediscretised coarse
 // operators down to 3 x 3 vertices) and set u += e.  The FP32 inner solve
 // needs only a few digits per correction; FP64 residuals recover the rest.
+//
+// The FP32 levels are stored with rows padded to P = N rounded up to 4
+// floats, so that the fine-level sweeps and residuals work on 16-byte pieces:
+// a lane owns 4 points of PY rows, loads the rows it needs (pieces at x-4, x,
+// x+4) through the caches, and writes whole rows -- interior values, and 0 at
+// the boundary and padding points, which those arrays hold anyway -- so that
+// no cache line is written in part.
 #include "kg_internal.h"
 
 #include <cmath>
@@ -23,19 +30,98 @@
 
 namespace {
 
-constexpr int PT = 16;   // 16 x 16 threads per workgroup
+constexpr int PT = 16;   // 16 x 16 threads per workgroup (scalar kernels)
+constexpr int VW = 4;    // waves per workgroup of the 4-point kernels, stacked in y
+constexpr int PY = 4;    // rows per lane of the 4-point kernels
 
-// damped Jacobi sweep (interior only; boundary rows/columns of uo untouched)
-template <class T>
-__global__ __launch_bounds__(PT * PT)
-void pj_smooth_k(int N, T c0, T c1, T c2, T cf, const T* __restrict__ u,
-                 const T* __restrict__ f, T* __restrict__ uo)
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+typedef __attribute__((address_space(1))) const f4 gf4;
+
+__device__ __forceinline__ f4 ldf4(const float* p) { return *(gf4*)p; }
+
+// The rows j0-1 .. j0+PY of a lane's 4-point column x (pieces at x-4, x, x+4
+// clamped into the padded row; clamped pieces feed only boundary points).
+struct Win4 {
+    float w[PY + 2][12];
+};
+__device__ __forceinline__ void load_win4(const float* __restrict__ u, int N, int P, int x, int j0,
+                                          Win4& W)
 {
-    const int i = blockIdx.x * PT + threadIdx.x, j = blockIdx.y * PT + threadIdx.y;
-    if (i < 1 || j < 1 || i >= N - 1 || j >= N - 1) return;
-    const long long o = (long long)j * N + i;
-    uo[o] = c0 * u[o] + c1 * (u[o - 1] + u[o - N] + u[o + 1] + u[o + N]) +
-            c2 * (u[o - 1 - N] + u[o - 1 + N] + u[o + 1 - N] + u[o + 1 + N]) + cf * f[o];
+    const int xl = x >= 4 ? x - 4 : x, xr = x + 4 < P ? x + 4 : x;
+#pragma unroll
+    for (int q = 0; q < PY + 2; q++) {
+        const float* row = u + (long long)min(j0 - 1 + q, N - 1) * P;
+        const f4 a = ldf4(row + xl), b = ldf4(row + x), c = ldf4(row + xr);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            W.w[q][v] = a[v];
+            W.w[q][4 + v] = b[v];
+            W.w[q][8 + v] = c[v];
+        }
+    }
+}
+
+// damped Jacobi sweep on 4-point pieces, u' = c0 u + c1 (W + S + E + N) +
+// c2 (SW + NW + SE + NE) + cf f; writes rows 1 .. N-2 whole (0 at the
+// boundary and padding)
+__global__ __launch_bounds__(64 * VW)
+void pj_smooth4_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
+                  const float* __restrict__ u, const float* __restrict__ f, float* __restrict__ uo)
+{
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + lane * 4;
+    const int j0 = 1 + ((int)(blockIdx.x / (unsigned)ntx) * VW + w) * PY;
+    if (x >= P || j0 >= N - 1) return;
+    Win4 W;
+    load_win4(u, N, P, x, j0, W);
+#pragma unroll
+    for (int r = 0; r < PY; r++) {
+        const int j = j0 + r;
+        if (j >= N - 1) break;
+        const f4 fv = ldf4(f + (long long)j * P + x);
+        f4 o;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+#define U(dx, dy) W.w[r + 1 + (dy)][4 + v + (dx)]
+            const float val = c0 * U(0, 0) + c1 * (U(-1, 0) + U(0, -1) + U(1, 0) + U(0, 1)) +
+                              c2 * (U(-1, -1) + U(-1, 1) + U(1, -1) + U(1, 1)) + cf * fv[v];
+#undef U
+            o[v] = (x + v >= 1 && x + v < N - 1) ? val : 0.f;
+        }
+        __builtin_nontemporal_store(o, (__attribute__((address_space(1))) f4*)(uo + (long long)j * P + x));
+    }
+}
+
+// r = f - A u (FP32) on 4-point pieces, same expression as pj_residual_k
+// (the FP64 residual of the dense fine grid); writes rows 1 .. N-2 whole (0
+// at the boundary and padding)
+__global__ __launch_bounds__(64 * VW)
+void pj_residual4_k(int N, int P, int ntx, float inv6h2, const float* __restrict__ u,
+                    const float* __restrict__ f, float* __restrict__ r)
+{
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + lane * 4;
+    const int j0 = 1 + ((int)(blockIdx.x / (unsigned)ntx) * VW + w) * PY;
+    if (x >= P || j0 >= N - 1) return;
+    Win4 W;
+    load_win4(u, N, P, x, j0, W);
+#pragma unroll
+    for (int q = 0; q < PY; q++) {
+        const int j = j0 + q;
+        if (j >= N - 1) break;
+        const f4 fv = ldf4(f + (long long)j * P + x);
+        f4 o;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+#define U(dx, dy) W.w[q + 1 + (dy)][4 + v + (dx)]
+            const float au = (20.f * U(0, 0) - 4.f * (U(-1, 0) + U(0, -1) + U(1, 0) + U(0, 1)) -
+                              (U(-1, -1) + U(-1, 1) + U(1, -1) + U(1, 1))) * inv6h2;
+#undef U
+            o[v] = (x + v >= 1 && x + v < N - 1) ? fv[v] - au : 0.f;
+        }
+        __builtin_nontemporal_store(o, (__attribute__((address_space(1))) f4*)(r + (long long)j * P + x));
+    }
 }
 
 // r = f - A u on the interior, 0 on the boundary; TA: arithmetic type
@@ -58,36 +144,65 @@ void pj_residual_k(int N, T inv6h2, const T* __restrict__ u, const T* __restrict
 
 // full weighting: fc(I,J) from rf around (2I, 2J); coarse boundary = 0
 __global__ __launch_bounds__(PT * PT)
-void pj_restrict_k(int Nc, const float* __restrict__ rf, float* __restrict__ fc)
+void pj_restrict_k(int Nc, int Pc, int Pf, const float* __restrict__ rf, float* __restrict__ fc)
 {
     const int I = blockIdx.x * PT + threadIdx.x, J = blockIdx.y * PT + threadIdx.y;
     if (I >= Nc || J >= Nc) return;
-    const long long oc = (long long)J * Nc + I;
+    const long long oc = (long long)J * Pc + I;
     if (I == 0 || J == 0 || I == Nc - 1 || J == Nc - 1) {
         fc[oc] = 0.f;
         return;
     }
-    const int Nf = 2 * Nc - 1;
+    const int Nf = Pf;   // row pitch of the fine level
     const long long o = (long long)(2 * J) * Nf + 2 * I;
     fc[oc] = (4.f * rf[o] + 2.f * (rf[o - 1] + rf[o + 1] + rf[o - Nf] + rf[o + Nf]) +
               (rf[o - 1 - Nf] + rf[o + 1 - Nf] + rf[o - 1 + Nf] + rf[o + 1 + Nf])) * 0.0625f;
 }
 
-// u_f += bilinear interpolation of e_c (interior of the fine grid)
-__global__ __launch_bounds__(PT * PT)
-void pj_prolong_k(int Nf, const float* __restrict__ ec, float* __restrict__ uf)
+// u_f += bilinear interpolation of e_c on 4-point pieces of fine rows
+// (x multiple of 4, so fine points x, x+2 sit on coarse columns x/2, x/2+1);
+// e = e_c at even/even points, the mean of the 2 or 4 coarse neighbours
+// elsewhere; writes rows 1 .. Nf-2 whole (0 at the
+// boundary and padding).  One row per lane, 4 rows per workgroup.
+__global__ __launch_bounds__(256)
+void pj_prolong4_k(int Nf, int Pf, int Pc, int ntx, const float* __restrict__ ec,
+                   float* __restrict__ uf)
 {
-    const int i = blockIdx.x * PT + threadIdx.x, j = blockIdx.y * PT + threadIdx.y;
-    if (i < 1 || j < 1 || i >= Nf - 1 || j >= Nf - 1) return;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + lane * 4;
+    const int j = 1 + (int)(blockIdx.x / (unsigned)ntx) * 4 + w;
+    if (x >= Pf || j >= Nf - 1) return;
     const int Nc = (Nf + 1) / 2;
-    const int I = i >> 1, J = j >> 1;
-    const long long oc = (long long)J * Nc + I;
-    float e;
-    if (!(i & 1) && !(j & 1)) e = ec[oc];
-    else if (i & 1 && !(j & 1)) e = 0.5f * (ec[oc] + ec[oc + 1]);
-    else if (!(i & 1)) e = 0.5f * (ec[oc] + ec[oc + Nc]);
-    else e = 0.25f * (ec[oc] + ec[oc + 1] + ec[oc + Nc] + ec[oc + Nc + 1]);
-    uf[(long long)j * Nf + i] += e;
+    const int I0 = x >> 1, J = j >> 1;
+    // coarse columns I0 .. I0+2 (clamped; clamped ones feed boundary or
+    // padding points only) of rows J and J+1 (odd j)
+    float a[3], b[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int I = min(I0 + q, Nc - 1);
+        a[q] = ec[(long long)J * Pc + I];
+        b[q] = (j & 1) ? ec[(long long)(J + 1) * Pc + I] : 0.f;
+    }
+    float* row = uf + (long long)j * Pf + x;
+    const f4 u = ldf4(row);
+    f4 o;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int q = v >> 1;   // coarse column I0 + q of fine point x + v
+        float e;
+        if (!(v & 1) && !(j & 1)) e = a[q];
+        else if ((v & 1) && !(j & 1)) e = 0.5f * (a[q] + a[q + 1]);
+        else if (!(v & 1)) e = 0.5f * (a[q] + b[q]);
+        else e = 0.25f * (a[q] + a[q + 1] + b[q] + b[q + 1]);
+        o[v] = (x + v >= 1 && x + v < Nf - 1) ? u[v] + e : 0.f;
+    }
+    __builtin_nontemporal_store(o, (__attribute__((address_space(1))) f4*)row);
+}
+
+// exact solve on the coarsest level (3 x 3 vertices: one unknown)
+__global__ void pj_exact3_k(int P, float s, const float* __restrict__ f, float* __restrict__ u)
+{
+    if (threadIdx.x == 0) u[P + 1] = s * f[P + 1];
 }
 
 __global__ __launch_bounds__(256)
@@ -97,22 +212,30 @@ void pj_zero_k(long long n, float* __restrict__ p)
         p[e] = 0.f;
 }
 
+// dense N x N FP64 -> FP32 rows of pitch P (padding 0)
 __global__ __launch_bounds__(256)
-void pj_to_f32_k(long long n, const double* __restrict__ s, float* __restrict__ d)
+void pj_to_f32_k(int N, int P, const double* __restrict__ s, float* __restrict__ d)
 {
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256)
-        d[e] = (float)s[e];
+    const long long n = (long long)N * P;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+        const long long j = e / P, i = e - j * P;
+        d[e] = i < N ? (float)s[j * N + i] : 0.f;
+    }
 }
 
+// dense N x N FP64 u += FP32 rows of pitch P
 __global__ __launch_bounds__(256)
-void pj_add_k(long long n, const float* __restrict__ e, double* __restrict__ u)
+void pj_add_k(int N, int P, const float* __restrict__ e, double* __restrict__ u)
 {
-    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256)
-        u[q] += (double)e[q];
+    const long long n = (long long)N * N;
+    for (long long q = (long long)blockIdx.x * 256 + threadIdx.x; q < n; q += (long long)gridDim.x * 256) {
+        const long long j = q / N, i = q - j * N;
+        u[q] += (double)e[j * P + i];
+    }
 }
 
 struct Level {
-    int N;
+    int N, P;   // vertices per side, row pitch (N rounded up to 4)
     double h;
     float *u, *t, *f, *r;
 };
@@ -120,16 +243,30 @@ struct Level {
 dim3 grid2(int N) { return dim3((N + PT - 1) / PT, (N + PT - 1) / PT); }
 unsigned blocks1(long long n) { return (unsigned)std::min<long long>((n + 255) / 256, 256 * 64); }
 
+// workgroups of the 4-point kernels on level L
+struct Grid4 {
+    int ntx;
+    unsigned blocks;
+};
+Grid4 grid4(const Level& L)
+{
+    Grid4 g;
+    g.ntx = (L.P + 255) / 256;
+    g.blocks = (unsigned)g.ntx * (unsigned)((L.N - 2 + VW * PY - 1) / (VW * PY));
+    return g;
+}
+
 // n damped-Jacobi sweeps on level L (result in L.u)
 void smooth(hipStream_t st, Level& L, int n, double omega)
 {
     const float c0 = (float)(1 - omega), c1 = (float)(omega / 5), c2 = (float)(omega / 20);
     const float cf = (float)(3 * omega * L.h * L.h / 10);
+    const Grid4 g = grid4(L);
     for (int s = 0; s < n; s++) {
-        hipLaunchKernelGGL(pj_smooth_k<float>, grid2(L.N), dim3(PT, PT), 0, st, L.N, c0, c1, c2, cf,
-                           L.u, L.f, L.t);
-        // the boundary of t must be 0 like u's: copy-free by swapping after
-        // both were zeroed at the start of the cycle
+        hipLaunchKernelGGL(pj_smooth4_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx, c0,
+                           c1, c2, cf, L.u, L.f, L.t);
+        // the boundary rows of t are 0 like u's (both zeroed at the start of
+        // the cycle, never written), the boundary columns written as 0
         std::swap(L.u, L.t);
     }
 }
@@ -138,19 +275,26 @@ void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post,
 {
     Level& L = lv[l];
     if (l + 1 == lv.size()) {
-        smooth(st, L, 32, omega);
+        // 3 x 3 vertices: (A u)(1,1) = 20 u / (6 h^2) = f
+        hipLaunchKernelGGL(pj_exact3_k, dim3(1), dim3(64), 0, st, L.P,
+                           (float)(6.0 * L.h * L.h / 20.0), L.f, L.u);
         return;
     }
     smooth(st, L, pre, omega);
-    hipLaunchKernelGGL((pj_residual_k<float, float>), grid2(L.N), dim3(PT, PT), 0, st, L.N,
+    const Grid4 g = grid4(L);
+    hipLaunchKernelGGL(pj_residual4_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx,
                        (float)(1.0 / (6.0 * L.h * L.h)), L.u, L.f, L.r);
     Level& C = lv[l + 1];
-    hipLaunchKernelGGL(pj_restrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, L.r, C.f);
-    const long long nc = (long long)C.N * C.N;
+    hipLaunchKernelGGL(pj_restrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P, L.r, C.f);
+    const long long nc = (long long)C.N * C.P;
     hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nc)), dim3(256), 0, st, nc, C.u);
     hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nc)), dim3(256), 0, st, nc, C.t);
     vcycle(st, lv, l + 1, pre, post, omega);
-    hipLaunchKernelGGL(pj_prolong_k, grid2(L.N), dim3(PT, PT), 0, st, L.N, C.u, L.u);
+    {
+        const int ntx = (L.P + 255) / 256;
+        const unsigned blocks = (unsigned)ntx * (unsigned)((L.N - 2 + 3) / 4);
+        hipLaunchKernelGGL(pj_prolong4_k, dim3(blocks), dim3(256), 0, st, L.N, L.P, C.P, ntx, C.u, L.u);
+    }
     smooth(st, L, post, omega);
 }
 
@@ -174,7 +318,8 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
     const double omega = 0.8;
     const int pre = 2, post = 2;
     std::vector<Level> lv;
-    for (int m = n; m >= 2; m /= 2) lv.push_back(Level{m + 1, 1.0 / m, nullptr, nullptr, nullptr, nullptr});
+    for (int m = n; m >= 2; m /= 2)
+        lv.push_back(Level{m + 1, (m + 1 + 3) / 4 * 4, 1.0 / m, nullptr, nullptr, nullptr, nullptr});
     std::vector<void*> allocs;
     auto fail = [&](int rc) {
         (void)hipStreamSynchronize(st);
@@ -191,14 +336,16 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
         return KG_OK;
     };
     for (auto& L : lv) {
-        const size_t b = (size_t)L.N * L.N * sizeof(float);
-        for (float** p : {&L.u, &L.t, &L.f, &L.r})
-            if (int rc = alloc(b, (void**)p)) return fail(rc);
+        const long long ne = (long long)L.N * L.P;
+        for (float** p : {&L.u, &L.t, &L.f, &L.r}) {
+            if (int rc = alloc((size_t)ne * sizeof(float), (void**)p)) return fail(rc);
+            hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(ne)), dim3(256), 0, st, ne, *p);
+        }
     }
     double* r64 = nullptr;
     if (int rc = alloc((size_t)np * sizeof(double), (void**)&r64)) return fail(rc);
 
-    kg_register_kernel("poisson_smooth", (const void*)pj_smooth_k<float>);
+    kg_register_kernel("poisson_smooth", (const void*)pj_smooth4_k);
     double st_f[5], st_r[5];
     if (int rc = kg_field_stats(ctx, KG_F64, (size_t)np, f, KG_F64, nullptr, st_f)) return fail(rc);
     const double fnorm = std::sqrt(st_f[1]);
@@ -217,12 +364,14 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
         if (rel < tol || outer >= max_outer) break;
         // inner FP32 solve of A e = r, e0 = 0
         Level& F = lv[0];
-        hipLaunchKernelGGL(pj_to_f32_k, dim3(blocks1(np)), dim3(256), 0, st, np, (const double*)r64, F.f);
-        hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(np)), dim3(256), 0, st, np, F.u);
-        hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(np)), dim3(256), 0, st, np, F.t);
+        const long long nf = (long long)F.N * F.P;
+        hipLaunchKernelGGL(pj_to_f32_k, dim3(blocks1(nf)), dim3(256), 0, st, F.N, F.P,
+                           (const double*)r64, F.f);
+        hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nf)), dim3(256), 0, st, nf, F.u);
+        hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nf)), dim3(256), 0, st, nf, F.t);
         for (int c = 0; c < vcycles; c++) vcycle(st, lv, 0, pre, post, omega);
         cycles += vcycles;
-        hipLaunchKernelGGL(pj_add_k, dim3(blocks1(np)), dim3(256), 0, st, np, (const float*)F.u, u);
+        hipLaunchKernelGGL(pj_add_k, dim3(blocks1(np)), dim3(256), 0, st, F.N, F.P, (const float*)F.u, u);
         outer++;
     }
     (void)hipEventRecord(e1, st);
@@ -243,4 +392,4 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
                                                  "%.3e after %d corrections (tol %.1e)", rel, outer, tol));
 }
 
-int kg_preload_poisson() { return kg_preload_kernel((const void*)pj_smooth_k<float>); }
+int kg_preload_poisson() { return kg_preload_kernel((const void*)pj_smooth4_k); }
