@@ -97,6 +97,35 @@ __device__ __forceinline__ unsigned kg_xcd_swizzle(unsigned b, unsigned n)
 
 #define KG_NUM_XCD 8
 
+// kg_xcd_chunk deals runs of `chunk` consecutive logical tiles to the XCDs in
+// turn: XCD x receives logical chunks x, x+8, x+16, ..., so at any moment
+// the 8 XCDs work on 8 consecutive chunks (the chip streams one window) and
+// the tiles of a chunk share one L2.  Ids past the last whole round of 8
+// chunks map to themselves.
+__device__ __forceinline__ unsigned kg_xcd_chunk(unsigned b, unsigned n, unsigned chunk)
+{
+    if (!KG_XCD_SWIZZLE) return b;
+    const unsigned round = chunk * KG_NUM_XCD;
+    if (b >= n / round * round) return b;
+    const unsigned x = b % KG_NUM_XCD, s = b / KG_NUM_XCD;   // XCD, its sequence number
+    return ((s / chunk) * KG_NUM_XCD + x) * chunk + s % chunk;
+}
+
+// Logical tile id of workgroup b of n, for tiles numbered x fastest with ntx
+// tiles per tile row.  ORD 0: hardware order (consecutive tiles on different
+// XCDs; the chip streams one window).  ORD 1: each XCD takes a contiguous
+// eighth (kg_xcd_swizzle).  ORD >= 2: chunks of ORD tile rows per XCD
+// (kg_xcd_chunk): a tile's x-neighbours and most of its y-neighbours share
+// its L2, and the chip still streams one window of 8 * ORD tile rows.
+template <int ORD>
+__device__ __forceinline__ unsigned kg_tile_order(unsigned b, unsigned n, unsigned ntx)
+{
+    static_assert(ORD >= 0, "tile order");
+    if constexpr (ORD == 0) return b;
+    else if constexpr (ORD == 1) return kg_xcd_swizzle(b, n);
+    else return kg_xcd_chunk(b, n, ntx * (unsigned)ORD);
+}
+
 // f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}):
 // a loop unrolled at compile time with the index usable as a constant.
 template <class Fn, int... I>

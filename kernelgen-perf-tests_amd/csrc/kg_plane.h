@@ -607,16 +607,15 @@ void rows2_kernel(int nx, int ny, int ntx, int nty, Op op)
 }
 
 // Single-grid launch in the dense layout: one workgroup per tile, tiles
-// enumerated x fastest, then y, then z-chunk, through the XCD swizzle when
-// SWZ (halo rows then hit in the XCD's L2; without it the chip streams one
-// contiguous window, which suits kernels with few LDS-served neighbours).
+// enumerated x fastest, then y, then z-chunk, in tile order ORD
+// (kg_tile_order: hardware, XCD-contiguous or chunks of ORD tile rows).
 // z-interior [ZB0, ns-ZB1), by default the same as the xy-interior.
-template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI, bool SWZ = true>
+template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI, int ORD = 1>
 __global__ __launch_bounds__(64 * C::W)
 void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
 {
     __shared__ typename C::F lds[C::LDS_BYTES / sizeof(typename C::F)];
-    unsigned b = SWZ ? kg_xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
+    unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
     const int bx = (int)(b % (unsigned)ntx);
     b /= (unsigned)ntx;
     const int by = (int)(b % (unsigned)nty);

@@ -268,13 +268,15 @@ static inline DGrid<C> make_dgrid(int nx, int ny, int lo, int hi)
 }
 
 // Requires an even nx and 16-byte aligned fields (checked by the host).
-template <class C, int LO, int HI, class Op>
+// ORD: tile order (kg_tile_order).
+template <class C, int LO, int HI, class Op, int ORD = 0>
 __global__ __launch_bounds__(64 * C::ROWS)
 void direct_kernel(int nx, int ny, int ntx, Op op)
 {
     const int lane = (int)(threadIdx.x & 63), r = (int)(threadIdx.x >> 6);
-    const int bx = (int)(blockIdx.x % (unsigned)ntx);
-    const int by = (int)(blockIdx.x / (unsigned)ntx);
+    const unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
+    const int bx = (int)(b % (unsigned)ntx);
+    const int by = (int)(b / (unsigned)ntx);
     const int x = bx * C::TXW + 2 * lane;
     const int j0 = LO + (by * C::ROWS + r) * C::PY;
     if (x >= nx || j0 >= ny - HI) return;   // no barriers in this kernel

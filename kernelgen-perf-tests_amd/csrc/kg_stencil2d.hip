@@ -190,13 +190,23 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fi
 #ifndef KG_G25_ROWS
 #define KG_G25_ROWS 4
 #endif
+// Tile order of the direct kernels (kg_tile_order), measured at 512x65536.
+#ifndef KG_ORD_JAC
+#define KG_ORD_JAC 4
+#endif
+#ifndef KG_ORD_GAUSS
+#define KG_ORD_GAUSS 4
+#endif
+#ifndef KG_ORD_LIFE
+#define KG_ORD_LIFE 4
+#endif
 using J9D = kgr::DCfg<KG_J9_PY, KG_J9_ROWS, 1>;
 using G25D = kgr::DCfg<KG_G25_PY, KG_G25_ROWS, 2>;
 
 // Single-field stencils: the direct march when nx is even and both fields
 // are 16-byte aligned, else the row engine.
 template <class DC, template <int> class CfgT, template <class> class OpT, int LO, int HI,
-          class Fill>
+          int ORD, class Fill>
 static int launch_direct(kg_ctx* ctx, const char* name, int nx, int ny, const void* in,
                          const void* out, Fill fill)
 {
@@ -208,7 +218,7 @@ static int launch_direct(kg_ctx* ctx, const char* name, int nx, int ny, const vo
     Op op{};
     fill(op);
     const kgr::DGrid<DC> g = kgr::make_dgrid<DC>(nx, ny, LO, HI);
-    auto kern = kgr::direct_kernel<DC, LO, HI, Op>;
+    auto kern = kgr::direct_kernel<DC, LO, HI, Op, ORD>;
     kg_register_kernel(name, (const void*)kern);
     if (int rc = kg_launch_begin(ctx, name)) return rc;
     if (!g.empty)
@@ -227,7 +237,7 @@ extern "C" int kg_jacobi(kg_ctx* ctx, int nx, int ny, double c0, double c1, doub
 {
     KG_REQUIRE(w0 && w1, "kg_jacobi: null buffer");
     KG_REQUIRE(w0 != w1, "kg_jacobi: w0 and w1 must be distinct");
-    return launch_direct<J9D, J9Cfg, JacobiOp, 1, 1>(ctx, "jacobi_", nx, ny, w0, w1, [&](auto& op) {
+    return launch_direct<J9D, J9Cfg, JacobiOp, 1, 1, KG_ORD_JAC>(ctx, "jacobi_", nx, ny, w0, w1, [&](auto& op) {
         op.src[0] = w0;
         op.dst = w1;
         op.c0 = c0;
@@ -242,7 +252,7 @@ extern "C" int kg_gaussblur(kg_ctx* ctx, int nx, int ny, double s0, double s1, d
     KG_REQUIRE(w0 && w1, "kg_gaussblur: null buffer");
     KG_REQUIRE(w0 != w1, "kg_gaussblur: w0 and w1 must be distinct");
     const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
-    return launch_direct<G25D, G25Cfg, GaussblurOp, 2, 2>(ctx, "gaussblur", nx, ny, w0, w1, [&](auto& op) {
+    return launch_direct<G25D, G25Cfg, GaussblurOp, 2, 2, KG_ORD_GAUSS>(ctx, "gaussblur", nx, ny, w0, w1, [&](auto& op) {
         op.src[0] = w0;
         op.dst = w1;
         op.f = f;
@@ -259,7 +269,7 @@ extern "C" int kg_gameoflife(kg_ctx* ctx, int nx, int ny, const double* u0, doub
 {
     KG_REQUIRE(u0 && u1, "kg_gameoflife: null buffer");
     KG_REQUIRE(u0 != u1, "kg_gameoflife: u0 and u1 must be distinct");
-    return launch_direct<J9D, J9Cfg, GameoflifeOp, 1, 1>(ctx, "gameoflife", nx, ny, u0, u1, [&](auto& op) {
+    return launch_direct<J9D, J9Cfg, GameoflifeOp, 1, 1, KG_ORD_LIFE>(ctx, "gameoflife", nx, ny, u0, u1, [&](auto& op) {
         op.src[0] = u0;
         op.dst = u1;
     });

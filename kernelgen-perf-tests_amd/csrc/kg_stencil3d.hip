@@ -17,6 +17,27 @@
 
 #include <type_traits>
 
+// Tile order per kernel (kg_tile_order: 0 hardware, 1 XCD-contiguous, R >= 2
+// chunks of R tile rows per XCD), measured at 512x256x256.
+#ifndef KG_ORD_LAP
+#define KG_ORD_LAP 2
+#endif
+#ifndef KG_ORD_GRAD
+#define KG_ORD_GRAD 2
+#endif
+#ifndef KG_ORD_WAVE
+#define KG_ORD_WAVE 1
+#endif
+#ifndef KG_ORD_GSRB
+#define KG_ORD_GSRB 0
+#endif
+#ifndef KG_ORD_DIV
+#define KG_ORD_DIV 0
+#endif
+#ifndef KG_ORD_UXX
+#define KG_ORD_UXX 0
+#endif
+
 namespace {
 
 using kgops::LaplacianOp;
@@ -80,7 +101,7 @@ void divergence2_k(int nx, int ny, int ns, int ntx, int nty, double alpha, doubl
                    double gamma, double* __restrict__ u, const double* __restrict__ ux,
                    const double* __restrict__ uy, const double* __restrict__ uz)
 {
-    unsigned b = blockIdx.x;
+    unsigned b = kg_tile_order<KG_ORD_DIV>(blockIdx.x, gridDim.x, (unsigned)ntx);
     const int bx = (int)(b % (unsigned)ntx);
     b /= (unsigned)ntx;
     const int by = (int)(b % (unsigned)nty);
@@ -182,7 +203,7 @@ void uxx1_2_k(int nx, int ny, int ns, int ntx, int nty, double c1, double c2, do
               const double* __restrict__ d1, const double* __restrict__ xx,
               const double* __restrict__ xy, const double* __restrict__ xz)
 {
-    unsigned b = blockIdx.x;
+    unsigned b = kg_tile_order<KG_ORD_UXX>(blockIdx.x, gridDim.x, (unsigned)ntx);
     const int bx = (int)(b % (unsigned)ntx);
     b /= (unsigned)ntx;
     const int by = (int)(b % (unsigned)nty);
@@ -309,13 +330,25 @@ struct LapgsrbOp {
     }
 };
 
-// Tile shapes from sweeps at 512x256x256 (cold caches, 40-launch means), with
-// the prefetch ring kept in flight (rotating slots, buffer loads): laplacian
-// W=8/D=2/KC=8 101 us (KC=16 104, W=16 113); wave13pt W=12/D=3 141 us (W=16
-// 148, W=14 169); lapgsrb with 8-byte lanes, W=12/D=2 137 us (W=16 157; with
-// 16-byte lanes the 25-point body needs too many registers: 307 us).
+// Tile shapes from sweeps at 512x256x256 (cold caches, 10-launch means), with
+// the prefetch ring kept in flight (rotating slots, buffer loads): laplacian in
+// chunks of 2 tile rows per XCD, W=12/D=2/KC=8 100 us (W=8 102, W=16 102,
+// D=3 102, KC=16 105, KC=4 107; XCD-contiguous order W=8 106); wave13pt
+// W=12/D=3 141 us (W=16 148, W=14 169); lapgsrb with 8-byte lanes, W=12/D=2,
+// hardware order 136 us (W=16 157; chunked orders 136-142; 16-byte lanes,
+// 130 VGPRs: 142-145 at W=12/16).
 //                 VEC W   H  LL LR ZL ZR D  NAUX KC
-template <int V> using LapCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 8>;
+#ifndef KG_LAP_W
+#define KG_LAP_W 12
+#endif
+#ifndef KG_LAP_D
+#define KG_LAP_D 2
+#endif
+#ifndef KG_LAP_KC
+#define KG_LAP_KC 8
+#endif
+template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0, KG_LAP_KC>;
+template <int V> using GradCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 8>;
 template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #ifndef KG_GSRB_W
 #define KG_GSRB_W 12
@@ -330,8 +363,8 @@ template <int V> using GsrbCfg = kgp::Cfg<1, KG_GSRB_W, 2, 1, 1, 2, 2, KG_GSRB_D
 
 // Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx and
 // 16-byte aligned fields (al).
-// SWZ: XCD-contiguous tile order (measured per kernel at 512x256x256).
-template <template <int> class CfgT, template <class> class OpT, int LO, int HI, bool SWZ = true,
+// ORD: tile order (kg_tile_order).
+template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int ORD = 1,
           class Fill>
 static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
@@ -345,7 +378,7 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
         Op op{};
         fill(op);
         const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI);
-        auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, SWZ>;
+        auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
@@ -379,7 +412,7 @@ extern "C" int kg_laplacian(kg_ctx* ctx, int nx, int ny, int ns, double alpha, d
 {
     KG_REQUIRE(w0 && w1, "kg_laplacian: null buffer");
     KG_REQUIRE(w0 != w1, "kg_laplacian: w0 and w1 must be distinct (out-of-place update)");
-    return launch_plane<LapCfg, LaplacianOp, 1, 1>(ctx, "laplacian", nx, ny, ns,
+    return launch_plane<LapCfg, LaplacianOp, 1, 1, KG_ORD_LAP>(ctx, "laplacian", nx, ny, ns,
                                                    kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
@@ -414,7 +447,7 @@ extern "C" int kg_gradient(kg_ctx* ctx, int nx, int ny, int ns, double alpha, do
     KG_REQUIRE(u && ux && uy && uz, "kg_gradient: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz && ux != uy && ux != uz && uy != uz,
                "kg_gradient: buffers must be distinct");
-    return launch_plane<LapCfg, GradientOp, 1, 1, false>(ctx, "gradient", nx, ny, ns,
+    return launch_plane<GradCfg, GradientOp, 1, 1, KG_ORD_GRAD>(ctx, "gradient", nx, ny, ns,
                                                          kg_aligned16(u, ux, uy, uz), [&](auto& op) {
         op.src = u;
         op.ux = ux;
@@ -431,7 +464,7 @@ extern "C" int kg_wave13pt(kg_ctx* ctx, int nx, int ny, int ns, double m0, doubl
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt: w2 aliases an input");
-    return launch_plane<WaveCfg, Wave13ptOp, 2, 2>(ctx, "wave13pt", nx, ny, ns,
+    return launch_plane<WaveCfg, Wave13ptOp, 2, 2, KG_ORD_WAVE>(ctx, "wave13pt", nx, ny, ns,
                                                    kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
@@ -471,7 +504,7 @@ extern "C" int kg_lapgsrb(kg_ctx* ctx, int nx, int ny, int ns, double c0, double
 {
     KG_REQUIRE(w0 && w1, "kg_lapgsrb: null buffer");
     KG_REQUIRE(w0 != w1, "kg_lapgsrb: w0 and w1 must be distinct");
-    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, false>(ctx, "lapgsrb", nx, ny, ns,
+    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, KG_ORD_GSRB>(ctx, "lapgsrb", nx, ny, ns,
                                                          kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
