@@ -26,6 +26,9 @@
 
 namespace {
 
+#ifndef KG_ORD_LIFE8
+#define KG_ORD_LIFE8 8                        // tile order (kg_tile_order)
+#endif
 #ifndef LIFE_WAVES
 #define LIFE_WAVES 4                          // segments per workgroup
 #endif
@@ -121,13 +124,15 @@ void life_u8_cells_k(int nx, int ny, int ntx, const uint8_t* __restrict__ c0,
 }
 
 // nx % 16 == 0 and 1 <= nx <= 2^24 (offsets within a band fit 31 bits).
-// Workgroups in hardware order (no XCD swizzle): bands of 4 rows measured
-// fastest at 4096^2 .. 16384^2 (8192^2: 32 us; 16-row bands 37 us, with the
-// swizzle 38 us), the whole chip then streaming one contiguous window.
+// Bands of 4 rows measured fastest at 4096^2 .. 16384^2 (16-row bands 15 %
+// slower).  Tile order KG_ORD_LIFE8: chunks of 8 bands per XCD, so a band's
+// neighbour rows and segments share its L2 (8192^2, 20-launch means: 25.2 us;
+// hardware order 29.0, XCD-contiguous 25.6, chunks of 2 or 4 bands 27.4 and
+// 26.3).
 __global__ __launch_bounds__(64 * LIFE_WAVES)
 void life_u8_k(int nx, int ny, int nsx, const uint8_t* __restrict__ c0, uint8_t* __restrict__ c1)
 {
-    const unsigned b = blockIdx.x;
+    const unsigned b = kg_tile_order<KG_ORD_LIFE8>(blockIdx.x, gridDim.x, (unsigned)nsx);
     const int sx = (int)(b % (unsigned)nsx);
     const int band = (int)(b / (unsigned)nsx);
     const int lane = (int)(threadIdx.x & 63);

@@ -594,12 +594,13 @@ __device__ __forceinline__ void rows2_tile(int nx, int ny, long long sy, long lo
 }
 
 // Single-grid launch of the direct row body over every plane of a dense
-// grid: tiles enumerated x fastest, then y, then plane, in hardware order.
-template <class C, int LO, int HI, class Op, int PY>
+// grid: tiles enumerated x fastest, then y, then plane, in tile order ORD
+// (kg_tile_order).
+template <class C, int LO, int HI, class Op, int PY, int ORD = 0>
 __global__ __launch_bounds__(64 * C::W)
 void rows2_kernel(int nx, int ny, int ntx, int nty, Op op)
 {
-    const unsigned b = blockIdx.x;
+    const unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
     const int bx = (int)(b % (unsigned)ntx);
     const int by = (int)((b / (unsigned)ntx) % (unsigned)nty);
     const int k = (int)(b / ((unsigned)ntx * (unsigned)nty));

@@ -16,6 +16,18 @@
 #include <algorithm>
 #include <type_traits>
 
+// Tile orders (kg_tile_order) of the FP32/BF16 plane-engine kernels and of the
+// per-plane direct row body, measured at 512^3 (12-launch means): FP32
+// laplacian 223 us XCD-contiguous, 210 in chunks of 2 tile rows (213-211 in
+// hardware order or chunks of 4, 8), FP32 wave13pt 337 -> 315 us, BF16 within
+// 3 %; per-plane gaussblur 242 us (FP32) in hardware order, 251-280 otherwise.
+#ifndef KG_ORD_TYPED
+#define KG_ORD_TYPED 2
+#endif
+#ifndef KG_ORD_ROWS2
+#define KG_ORD_ROWS2 0
+#endif
+
 namespace {
 
 using namespace kgops;
@@ -40,7 +52,7 @@ static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
         Op op{};
         fill(op);
         const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI, ZLO, ZHI, KC);
-        auto kern = kgp::plane_kernel<C, LO, HI, Op, ZLO, ZHI>;
+        auto kern = kgp::plane_kernel<C, LO, HI, Op, ZLO, ZHI, KG_ORD_TYPED>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
@@ -72,7 +84,7 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
     fill(op);
     const int nxi = nx - LO - HI, nyi = ny - LO - HI;
     const int ntx = (nx + R2::TXW - 1) / R2::TXW, nty = nyi > 0 ? (nyi + R2::TY - 1) / R2::TY : 0;
-    auto kern = kgp::rows2_kernel<C, LO, HI, Op, PY>;
+    auto kern = kgp::rows2_kernel<C, LO, HI, Op, PY, KG_ORD_ROWS2>;
     kg_register_kernel(name, (const void*)kern);
     if (int rc = kg_launch_begin(ctx, name)) return rc;
     if (nxi > 0 && nyi > 0 && ns > 0)
