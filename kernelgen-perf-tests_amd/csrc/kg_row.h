@@ -98,13 +98,14 @@ struct View {
 
 // Op supplies src[NF], side[NAUX] and run(view, o, x, xmask) which computes
 // and stores the lane's VEC points of row j (o = j*nx).
-template <class C, int LO, int HI, class Op, bool SWZ = true>
+// ORD: tile order (kg_tile_order).
+template <class C, int LO, int HI, class Op, int ORD = 1>
 __global__ __launch_bounds__(64 * C::W)
 void row_kernel(int nx, int ny, int ntx, Op op)
 {
     __shared__ double lds[C::NF * C::NLS * C::LW];
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    const unsigned b = SWZ ? kg_xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
+    const unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
     const int bx = (int)(b % (unsigned)ntx);
     const int by = (int)(b / (unsigned)ntx);
     const int x0 = bx * C::TXW;

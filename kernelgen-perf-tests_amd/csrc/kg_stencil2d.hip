@@ -150,7 +150,7 @@ template <int V> using WhCfg = kgr::Cfg<V, 8 / V, 1, 5, 0, 0, 1, 1, 2, 3, 8>;
 // Launch Op on the row engine; 16-byte lanes (VEC=2) need an even nx and
 // 16-byte aligned fields (al).
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int VEC_EVEN,
-          class Fill>
+          int ORD = 1, class Fill>
 static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
@@ -162,7 +162,7 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fi
         Op op{};
         fill(op);
         const kgr::Grid g = kgr::make_grid<C>(nx, ny, LO, HI);
-        auto kern = kgr::row_kernel<C, LO, HI, Op>;
+        auto kern = kgr::row_kernel<C, LO, HI, Op, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
@@ -175,11 +175,15 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fi
 }
 
 // Direct march shapes (output rows per lane, waves per workgroup) from
-// sweeps at 512x65536 (cold caches, against the row engine's 113-117 us):
-// jacobi PY=4/4 waves 102 us, gameoflife 103 us (PY=8: 102 / 106);
-// gaussblur PY=8/4 waves 100 us (PY=4: 110, 2 waves: 100-118).
+// sweeps at 512x65536 (cold caches, against the row engine's 113-117 us),
+// in chunked tile order: jacobi PY=8/4 waves 102 us (PY=4: 103, 8 waves
+// 104), gameoflife PY=4/4 waves 101 us (PY=8: 107); gaussblur PY=8/4 waves
+// 95 us (PY=4/8 waves 111, PY=8/8 waves 106).
 #ifndef KG_J9_PY
 #define KG_J9_PY 4
+#endif
+#ifndef KG_JAC_PY
+#define KG_JAC_PY 8
 #endif
 #ifndef KG_J9_ROWS
 #define KG_J9_ROWS 4
@@ -200,7 +204,11 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fi
 #ifndef KG_ORD_LIFE
 #define KG_ORD_LIFE 4
 #endif
+#ifndef KG_ORD_WH
+#define KG_ORD_WH 1
+#endif
 using J9D = kgr::DCfg<KG_J9_PY, KG_J9_ROWS, 1>;
+using JacD = kgr::DCfg<KG_JAC_PY, KG_J9_ROWS, 1>;
 using G25D = kgr::DCfg<KG_G25_PY, KG_G25_ROWS, 2>;
 
 // Single-field stencils: the direct march when nx is even and both fields
@@ -237,7 +245,7 @@ extern "C" int kg_jacobi(kg_ctx* ctx, int nx, int ny, double c0, double c1, doub
 {
     KG_REQUIRE(w0 && w1, "kg_jacobi: null buffer");
     KG_REQUIRE(w0 != w1, "kg_jacobi: w0 and w1 must be distinct");
-    return launch_direct<J9D, J9Cfg, JacobiOp, 1, 1, KG_ORD_JAC>(ctx, "jacobi_", nx, ny, w0, w1, [&](auto& op) {
+    return launch_direct<JacD, J9Cfg, JacobiOp, 1, 1, KG_ORD_JAC>(ctx, "jacobi_", nx, ny, w0, w1, [&](auto& op) {
         op.src[0] = w0;
         op.dst = w1;
         op.c0 = c0;
@@ -288,7 +296,7 @@ extern "C" int kg_whispering(kg_ctx* ctx, int nx, int ny, double mu, double epsi
     const long long n = (long long)nx * ny;
     const bool al = kg_aligned16(e_in, e_out, h0, h1, u_em0, u_em1, ca, cb, da, db) &&
                     kg_aligned16(e_in + n, e_out + n);
-    return launch_row<WhCfg, WhisperingOp, 1, 1, 2>(ctx, "whispering", nx, ny, al, [&](auto& op) {
+    return launch_row<WhCfg, WhisperingOp, 1, 1, 2, KG_ORD_WH>(ctx, "whispering", nx, ny, al, [&](auto& op) {
         op.src[0] = h0;
         op.src[1] = ca;
         op.src[2] = cb;
