@@ -20,6 +20,7 @@ struct kg_ctx {
     long long drained;                // launches already returned by kg_timing_drain
     const char* last_kernel;          // name of the most recent launch
     double* stats;                    // device partials of kg_field_stats
+    int cus;                          // compute units of the device
 };
 
 // Bytes of the context's kg_field_stats scratch.

@@ -607,13 +607,13 @@ void rows2_kernel(int nx, int ny, int ntx, int nty, Op op)
     rows2_tile<C, LO, HI, Op, PY>(nx, ny, nx, (long long)nx * ny, bx, by, k, op);
 }
 
-// Single-grid launch in the dense layout: one workgroup per tile, tiles
-// enumerated x fastest, then y, then z-chunk, in tile order ORD
+// Single-grid launch in the dense layout: one workgroup per tile of kc
+// planes, tiles enumerated x fastest, then y, then z-chunk, in tile order ORD
 // (kg_tile_order: hardware, XCD-contiguous or chunks of ORD tile rows).
 // z-interior [ZB0, ns-ZB1), by default the same as the xy-interior.
 template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI, int ORD = 1>
 __global__ __launch_bounds__(64 * C::W)
-void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
+void plane_kernel(int nx, int ny, int ns, int ntx, int nty, int kc, Op op)
 {
     __shared__ typename C::F lds[C::LDS_BYTES / sizeof(typename C::F)];
     unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
@@ -622,7 +622,7 @@ void plane_kernel(int nx, int ny, int ns, int ntx, int nty, Op op)
     const int by = (int)(b % (unsigned)nty);
     const int bz = (int)(b / (unsigned)nty);
     plane_tile<C, LO, HI, ZB0, ZB1, Op>(lds, nx, ny, ns, nx, (long long)nx * ny, bx, by, bz,
-                                        C::KC, op);
+                                        kc, op);
 }
 
 // Store the VEC results of a lane: one vector non-temporal store when all

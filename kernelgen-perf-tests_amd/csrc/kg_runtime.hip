@@ -128,6 +128,11 @@ extern "C" int kg_init(kg_ctx** out, int device)
     if (!c) return kg_set_error(KG_ERR_INVALID, "kg_init: host allocation failed");
     c->device = device;
     c->timing = 1;
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        c->cus <= 0) {
+        free(c);
+        return kg_set_error(KG_ERR_HIP, "kg_init: cannot read the compute unit count");
+    }
     c->ev_start = (hipEvent_t*)calloc(KG_TIMING_RING, sizeof(hipEvent_t));
     c->ev_stop = (hipEvent_t*)calloc(KG_TIMING_RING, sizeof(hipEvent_t));
     if (!c->ev_start || !c->ev_stop) {

@@ -15,6 +15,7 @@
 #include "kg_ops3d.h"
 #include "kg_zmarch.h"
 
+#include <algorithm>
 #include <type_traits>
 
 // Tile order per kernel (kg_tile_order: 0 hardware, 1 XCD-contiguous, R >= 2
@@ -334,9 +335,12 @@ struct LapgsrbOp {
 // the prefetch ring kept in flight (rotating slots, buffer loads): laplacian in
 // chunks of 2 tile rows per XCD, W=12/D=2/KC=8 100 us (W=8 102, W=16 102,
 // D=3 102, KC=16 105, KC=4 107; XCD-contiguous order W=8 106); wave13pt
-// W=12/D=3 141 us (W=16 148, W=14 169); lapgsrb with 8-byte lanes, W=12/D=2,
-// hardware order 136 us (W=16 157; chunked orders 136-142; 16-byte lanes,
-// 130 VGPRs: 142-145 at W=12/16).
+// W=12/D=3 141 us (W=16 148, W=14 169; z-chunks lengthened to 2 workgroups
+// per CU 172); lapgsrb with 8-byte lanes, W=12/D=2, hardware order, z-chunks
+// lengthened to about 2 workgroups per CU (126 planes here) 127-130 us (fixed
+// 32 planes 136, 64 130, 256 154; 1 workgroup per CU 154, 4 128; W=16 157,
+// W=14 183, D=3 179; chunked orders 136-142; 16-byte lanes, 130 VGPRs:
+// 142-145).
 //                 VEC W   H  LL LR ZL ZR D  NAUX KC
 #ifndef KG_LAP_W
 #define KG_LAP_W 12
@@ -359,13 +363,22 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #ifndef KG_GSRB_KC
 #define KG_GSRB_KC 32
 #endif
+#ifndef KG_GSRB_WGPC
+#define KG_GSRB_WGPC 2
+#endif
+#ifndef KG_WAVE_WGPC
+#define KG_WAVE_WGPC 0
+#endif
 template <int V> using GsrbCfg = kgp::Cfg<1, KG_GSRB_W, 2, 1, 1, 2, 2, KG_GSRB_D, 0, KG_GSRB_KC>;
 
 // Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx and
 // 16-byte aligned fields (al).
-// ORD: tile order (kg_tile_order).
+// ORD: tile order (kg_tile_order).  WGPC > 0: z-chunks longer than C::KC
+// planes when the launch would otherwise exceed about WGPC workgroups per
+// compute unit (each workgroup then marches further and fills its pipeline
+// fewer times); WGPC = 0 keeps C::KC.
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int ORD = 1,
-          class Fill>
+          int WGPC = 0, class Fill>
 static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
@@ -377,13 +390,23 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
         using Op = OpT<C>;
         Op op{};
         fill(op);
-        const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI);
+        int kc = C::KC;
+        if (WGPC > 0) {
+            const kgp::Grid g0 = kgp::make_grid<C>(nx, ny, ns, LO, HI);
+            const long long xy = (long long)g0.ntx * g0.nty, want = (long long)WGPC * ctx->cus;
+            if (!g0.empty && xy * g0.nkc > want) {
+                const long long chunks = want > xy ? want / xy : 1;
+                const long long nzi = ns - LO - HI;
+                kc = (int)std::max<long long>(C::KC, (nzi + chunks - 1) / chunks);
+            }
+        }
+        const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI, LO, HI, kc);
         auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
             hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), 0, ctx->stream,
-                               nx, ny, ns, g.ntx, g.nty, op);
+                               nx, ny, ns, g.ntx, g.nty, kc, op);
         return kg_launch_end(ctx);
     };
     if (nx % 2 == 0 && al) return go(std::integral_constant<int, 2>{});
@@ -464,7 +487,7 @@ extern "C" int kg_wave13pt(kg_ctx* ctx, int nx, int ny, int ns, double m0, doubl
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt: w2 aliases an input");
-    return launch_plane<WaveCfg, Wave13ptOp, 2, 2, KG_ORD_WAVE>(ctx, "wave13pt", nx, ny, ns,
+    return launch_plane<WaveCfg, Wave13ptOp, 2, 2, KG_ORD_WAVE, KG_WAVE_WGPC>(ctx, "wave13pt", nx, ny, ns,
                                                    kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
@@ -504,7 +527,7 @@ extern "C" int kg_lapgsrb(kg_ctx* ctx, int nx, int ny, int ns, double c0, double
 {
     KG_REQUIRE(w0 && w1, "kg_lapgsrb: null buffer");
     KG_REQUIRE(w0 != w1, "kg_lapgsrb: w0 and w1 must be distinct");
-    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, KG_ORD_GSRB>(ctx, "lapgsrb", nx, ny, ns,
+    return launch_plane<GsrbCfg, LapgsrbOp, 2, 2, KG_ORD_GSRB, KG_GSRB_WGPC>(ctx, "lapgsrb", nx, ny, ns,
                                                          kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;

@@ -57,7 +57,7 @@ static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
             hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), 0, ctx->stream,
-                               nx, ny, ns, g.ntx, g.nty, op);
+                               nx, ny, ns, g.ntx, g.nty, KC, op);
         return kg_launch_end(ctx);
     };
     if (nx % VMAX == 0 && al) return go(std::integral_constant<int, VMAX>{});

@@ -82,6 +82,19 @@ def test_gpu_matches_oracle_full_size_one_step(gpu_ctx, oracle, name):
         assert_bit_equal(got[role], want[role], f"{name} full nt=1 {role}")
 
 
+@pytest.mark.parametrize("dims", [(512, 520, 40), (512, 256, 200)])
+def test_gpu_lapgsrb_long_z_chunks(gpu_ctx, oracle, dims):
+    """lapgsrb lengthens its z-chunks when the launch would exceed about two
+    workgroups per compute unit: (512, 520, 40) makes one chunk of every
+    plane (520 tiles in xy), (512, 256, 200) two chunks of 98 planes."""
+    spec = K.SPECS["lapgsrb"]
+    coefs, arrays = K.random_case(spec, dims, 77)
+    want = oracle.run(spec, dims, coefs, arrays, 2)
+    got = K.GpuBackend(gpu_ctx).run(spec, dims, coefs, arrays, 2)
+    for role in want:
+        assert_bit_equal(got[role], want[role], f"lapgsrb {dims} {role}")
+
+
 @pytest.mark.parametrize("name", ["laplacian", "wave13pt", "whispering", "gameoflife", "uxx1"])
 def test_gpu_matches_oracle_ten_steps(gpu_ctx, oracle, name):
     """Ten ping-pong steps (the suite's nt) at a mid size."""
