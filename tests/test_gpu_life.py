@@ -15,7 +15,9 @@ SIZES = [(3, 3), (4, 5), (16, 3), (17, 12), (18, 18), (33, 5), (48, 40), (100, 3
          (4096, 19), (4113, 9), (8192, 33), (1040, 70),
          # band and segment edges of the vector kernel: 16-row bands, 1024-cell
          # wave segments, 4096-cell workgroups, a last band of one row
-         (32, 66), (2064, 100), (1024, 35), (5120, 34), (4112, 67)]
+         (32, 66), (2064, 100), (1024, 35), (5120, 34), (4112, 67),
+         # 150 workgroups: rounds of 8 chunks of 8 bands (128) plus a tail
+         (8192, 300), (4096, 600)]
 
 
 def _oracle_steps(oracle, board, nt, fill=0xAB):

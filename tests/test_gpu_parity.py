@@ -82,6 +82,23 @@ def test_gpu_matches_oracle_full_size_one_step(gpu_ctx, oracle, name):
         assert_bit_equal(got[role], want[role], f"{name} full nt=1 {role}")
 
 
+@pytest.mark.parametrize("name,dims", [("laplacian", (300, 200, 40)), ("gradient", (300, 200, 40)),
+                                       ("wave13pt", (300, 200, 40)), ("jacobi", (600, 1100)),
+                                       ("gaussblur", (600, 1100)), ("gameoflife", (600, 1100))])
+def test_gpu_tile_order_tails(gpu_ctx, oracle, name, dims):
+    """Grids whose workgroup count is not a whole number of tile-order rounds
+    (8 chunks of tile rows, or 8 XCD-contiguous parts): the workgroups past
+    the last whole round keep their hardware ids.  Launches here: laplacian
+    300 workgroups (rounds of 48), gradient 495 (48), wave13pt 150 (8),
+    jacobi and gaussblur 175 (160), gameoflife 345 (160)."""
+    spec = K.SPECS[name]
+    coefs, arrays = K.random_case(spec, dims, 88)
+    want = oracle.run(spec, dims, coefs, arrays, 2)
+    got = K.GpuBackend(gpu_ctx).run(spec, dims, coefs, arrays, 2)
+    for role in want:
+        assert_bit_equal(got[role], want[role], f"{name} {dims} {role}")
+
+
 @pytest.mark.parametrize("dims", [(512, 520, 40), (512, 256, 200)])
 def test_gpu_lapgsrb_long_z_chunks(gpu_ctx, oracle, dims):
     """lapgsrb lengthens its z-chunks when the launch would exceed about two

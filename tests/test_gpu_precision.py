@@ -87,6 +87,21 @@ def test_typed_matches_oracle(gpu_ctx, oracle, which, dtype):
 
 
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("wave13pt", "f32"),
+                                         ("laplacian", "bf16")])
+def test_typed_tile_order_tail(gpu_ctx, oracle, which, dtype):
+    """600x300x40: 270 (FP32 laplacian), 300 (BF16 laplacian) and 75
+    (wave13pt) workgroups, not whole rounds of 8 chunks of two tile rows; the
+    last ones keep their hardware ids."""
+    dims = (600, 300, 40)
+    fn = f"{which}_{dtype}"
+    ins, out, coefs, rot = _case(which, dtype, dims, 9)
+    want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 2, rot, f32)
+    got = _run_gpu(gpu_ctx, fn, dims, coefs, ins, out, 2, rot)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
+
+
+@pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("wave13pt", "f32"),
                                          ("laplacian", "bf16"), ("gaussblur3", "f32"),
                                          ("gaussblur3", "f64")])
 def test_typed_full_size_one_step(gpu_ctx, oracle, which, dtype):
