@@ -21,6 +21,7 @@ struct kg_ctx {
     const char* last_kernel;          // name of the most recent launch
     double* stats;                    // device partials of kg_field_stats
     int cus;                          // compute units of the device
+    int lds_cu;                       // LDS bytes per compute unit
 };
 
 // Bytes of the context's kg_field_stats scratch.

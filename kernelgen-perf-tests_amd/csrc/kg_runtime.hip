@@ -129,9 +129,11 @@ extern "C" int kg_init(kg_ctx** out, int device)
     c->device = device;
     c->timing = 1;
     if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-        c->cus <= 0) {
+        hipDeviceGetAttribute(&c->lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              device) != hipSuccess ||
+        c->cus <= 0 || c->lds_cu <= 0) {
         free(c);
-        return kg_set_error(KG_ERR_HIP, "kg_init: cannot read the compute unit count");
+        return kg_set_error(KG_ERR_HIP, "kg_init: cannot read the compute unit count or LDS size");
     }
     c->ev_start = (hipEvent_t*)calloc(KG_TIMING_RING, sizeof(hipEvent_t));
     c->ev_stop = (hipEvent_t*)calloc(KG_TIMING_RING, sizeof(hipEvent_t));

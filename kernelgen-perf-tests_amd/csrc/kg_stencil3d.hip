@@ -352,6 +352,9 @@ struct LapgsrbOp {
 #define KG_LAP_KC 8
 #endif
 template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0, KG_LAP_KC>;
+// gradient: the laplacian's first shape (W=8/D=2/KC=8) in chunks of 2 tile
+// rows, at most 2 workgroups (16 waves) per CU: 201 us (3 workgroups 208;
+// W=12 212, W=16 216, KC=16 224, D=3 209, XCD-contiguous 219).
 template <int V> using GradCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 8>;
 template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #ifndef KG_GSRB_W
@@ -376,9 +379,10 @@ template <int V> using GsrbCfg = kgp::Cfg<1, KG_GSRB_W, 2, 1, 1, 2, 2, KG_GSRB_D
 // ORD: tile order (kg_tile_order).  WGPC > 0: z-chunks longer than C::KC
 // planes when the launch would otherwise exceed about WGPC workgroups per
 // compute unit (each workgroup then marches further and fills its pipeline
-// fewer times); WGPC = 0 keeps C::KC.
+// fewer times); WGPC = 0 keeps C::KC.  OCC > 0: at most OCC resident
+// workgroups per compute unit, enforced by reserving unused dynamic LDS.
 template <template <int> class CfgT, template <class> class OpT, int LO, int HI, int ORD = 1,
-          int WGPC = 0, class Fill>
+          int WGPC = 0, int OCC = 0, class Fill>
 static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
@@ -401,11 +405,15 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
             }
         }
         const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI, LO, HI, kc);
+        // (OCC+1) workgroups of lds_cu/OCC - 256 bytes do not fit in lds_cu
+        size_t smem = 0;
+        if (OCC > 0 && (size_t)ctx->lds_cu / OCC > C::LDS_BYTES + 256)
+            smem = (size_t)ctx->lds_cu / OCC - C::LDS_BYTES - 256;
         auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
-            hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), 0, ctx->stream,
+            hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), smem, ctx->stream,
                                nx, ny, ns, g.ntx, g.nty, kc, op);
         return kg_launch_end(ctx);
     };
@@ -470,7 +478,7 @@ extern "C" int kg_gradient(kg_ctx* ctx, int nx, int ny, int ns, double alpha, do
     KG_REQUIRE(u && ux && uy && uz, "kg_gradient: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz && ux != uy && ux != uz && uy != uz,
                "kg_gradient: buffers must be distinct");
-    return launch_plane<GradCfg, GradientOp, 1, 1, KG_ORD_GRAD>(ctx, "gradient", nx, ny, ns,
+    return launch_plane<GradCfg, GradientOp, 1, 1, KG_ORD_GRAD, 0, 2>(ctx, "gradient", nx, ny, ns,
                                                          kg_aligned16(u, ux, uy, uz), [&](auto& op) {
         op.src = u;
         op.ux = ux;
