@@ -99,6 +99,17 @@ __device__ __forceinline__ unsigned kg_xcd_swizzle(unsigned b, unsigned n)
 
 #define KG_NUM_XCD 8
 
+// Dynamic LDS to reserve (unused) so that at most occ workgroups, each with
+// static_lds bytes of its own, are resident on one compute unit: occ+1
+// workgroups of lds_cu/occ - 256 bytes do not fit in lds_cu.  0 when occ <= 0
+// or the workgroup's own LDS already allows no more.
+inline size_t kg_occupancy_lds(const kg_ctx* ctx, int occ, size_t static_lds)
+{
+    if (occ <= 0) return 0;
+    const size_t per = (size_t)ctx->lds_cu / (size_t)occ;
+    return per > static_lds + 256 ? per - static_lds - 256 : 0;
+}
+
 // kg_xcd_chunk deals runs of `chunk` consecutive logical tiles to the XCDs in
 // turn: XCD x receives logical chunks x, x+8, x+16, ..., so at any moment
 // the 8 XCDs work on 8 consecutive chunks (the chip streams one window) and

@@ -81,14 +81,16 @@ void divergence_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double
 // (x even) of row j and PZ consecutive planes; every operand is a 16-byte
 // load through the caches (ux as the aligned pairs at x-2, x, x+2; uy at rows
 // j-1, j+1; uz at planes k0-1 .. k0+PZ), all issued before the first use.
-// Workgroups stay in hardware order.  Same expression as divergence_k.
+// Workgroups stay in hardware order, at most DIV_OCC (16 waves) per compute
+// unit: 183 us at 512x256x256 (unlimited 187, 3 per CU 196; uxx1 gains
+// nothing from such a cap).  Same expression as divergence_k.
 #ifndef KG_DIV_PZ
 #define KG_DIV_PZ 4
 #endif
 #ifndef KG_DIV_ROWS
 #define KG_DIV_ROWS 4
 #endif
-constexpr int DIV_PZ = KG_DIV_PZ, DIV_ROWS = KG_DIV_ROWS;
+constexpr int DIV_PZ = KG_DIV_PZ, DIV_ROWS = KG_DIV_ROWS, DIV_OCC = 4;
 
 typedef double d2v __attribute__((ext_vector_type(2)));
 
@@ -405,10 +407,7 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
             }
         }
         const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI, LO, HI, kc);
-        // (OCC+1) workgroups of lds_cu/OCC - 256 bytes do not fit in lds_cu
-        size_t smem = 0;
-        if (OCC > 0 && (size_t)ctx->lds_cu / OCC > C::LDS_BYTES + 256)
-            smem = (size_t)ctx->lds_cu / OCC - C::LDS_BYTES - 256;
+        const size_t smem = kg_occupancy_lds(ctx, OCC, C::LDS_BYTES);
         auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
@@ -465,7 +464,7 @@ extern "C" int kg_divergence(kg_ctx* ctx, int nx, int ny, int ns, double alpha, 
         kg_register_kernel("divergence", (const void*)divergence2_k);
         if (int rc = kg_launch_begin(ctx, "divergence")) return rc;
         if (!g.empty)
-            hipLaunchKernelGGL(divergence2_k, dim3(g.blocks), dim3(64 * DIV_ROWS), 0, ctx->stream,
+            hipLaunchKernelGGL(divergence2_k, dim3(g.blocks), dim3(64 * DIV_ROWS), kg_occupancy_lds(ctx, DIV_OCC, 0), ctx->stream,
                                nx, ny, ns, g.ntx, g.nty, alpha, beta, gamma, u, ux, uy, uz);
         return kg_launch_end(ctx);
     }
