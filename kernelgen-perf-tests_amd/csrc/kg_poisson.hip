@@ -11,7 +11,8 @@
 //
 // damped Jacobi: u' = (1-w) u + (w/5) sum_axis + (w/20) sum_diag + (3 w h^2 / 10) f.
 //
-// Outer loop (FP64): r = f - A u; stop when ||r||_2 / ||f||_2 < tol; else
+// Outer loop (FP64): r = f - A u (one pass that also writes r in FP32 as the
+// inner right-hand side and sums r^2); stop when ||r||_2 / ||f||_2 < tol; else
 // solve A e = r approximately in FP32 (V-cycles: full-weighting restriction,
 // bilinear prolongation, damped-Jacobi smoothing, rediscretised coarse
 // operators down to 3 x 3 vertices) and set u += e.  The FP32 inner solve
@@ -93,7 +94,7 @@ void pj_smooth4_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
     }
 }
 
-// r = f - A u (FP32) on 4-point pieces, same expression as pj_residual_k
+// r = f - A u (FP32) on 4-point pieces, same expression as pj_residual64_k
 // (the FP64 residual of the dense fine grid); writes rows 1 .. N-2 whole (0
 // at the boundary and padding)
 __global__ __launch_bounds__(64 * VW)
@@ -124,22 +125,77 @@ void pj_residual4_k(int N, int P, int ntx, float inv6h2, const float* __restrict
     }
 }
 
-// r = f - A u on the interior, 0 on the boundary; TA: arithmetic type
-template <class T, class TR>
-__global__ __launch_bounds__(PT * PT)
-void pj_residual_k(int N, T inv6h2, const T* __restrict__ u, const T* __restrict__ f,
-                   TR* __restrict__ r)
+// The outer correction's FP64 residual, fused with its FP32 copy and its
+// squared norm: r = f - (20 u - 4 (u_W + u_S + u_E + u_N) - (u_SW + u_NW +
+// u_SE + u_NE)) / (6 h^2), summed in that order in FP64, written as
+// FP32 rows of pitch P, 0 on the boundary and the padding; the sum of r^2
+// over the workgroup's points goes to part[blockIdx.x] (lanes, then waves,
+// then the 4 wave sums in order: a fixed order).  A workgroup of 4 waves
+// takes 256 consecutive columns of RES_RY rows; a lane marches its column
+// down the rows with the 3 x 3 neighbourhood in registers (one new row of 3
+// values per step, the side values from L1).
+constexpr int RES_RY = 8;
+
+__global__ __launch_bounds__(256)
+void pj_residual64_k(int N, int P, int ntx, double inv6h2, const double* __restrict__ u,
+                     const double* __restrict__ f, float* __restrict__ r, double* __restrict__ part)
 {
-    const int i = blockIdx.x * PT + threadIdx.x, j = blockIdx.y * PT + threadIdx.y;
-    if (i >= N || j >= N) return;
-    const long long o = (long long)j * N + i;
-    if (i == 0 || j == 0 || i == N - 1 || j == N - 1) {
-        r[o] = (TR)0;
-        return;
+    __shared__ double wsum[4];
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + w * 64 + lane;
+    const int j0 = (int)(blockIdx.x / (unsigned)ntx) * RES_RY;
+    const int j1 = min(j0 + RES_RY, N);
+    // clamped columns and rows: they only feed points that are not computed
+    const int xc = min(x, N - 1), xl = max(xc - 1, 0), xr = min(xc + 1, N - 1);
+    auto row = [&](int j, double (&v)[3]) {
+        const long long o = (long long)min(max(j, 0), N - 1) * N;
+        v[0] = u[o + xl];
+        v[1] = u[o + xc];
+        v[2] = u[o + xr];
+    };
+    double a[3], b[3], c[3];   // rows j-1, j, j+1 at columns x-1, x, x+1
+    row(j0 - 1, a);
+    row(j0, b);
+    double acc = 0.0;
+    for (int j = j0; j < j1; j++) {
+        row(j + 1, c);
+        float out = 0.f;
+        if (x >= 1 && x < N - 1 && j >= 1 && j < N - 1) {
+            const double au = (20.0 * b[1] - 4.0 * (b[0] + a[1] + b[2] + c[1]) -
+                               (a[0] + c[0] + a[2] + c[2])) * inv6h2;
+            const double rv = f[(long long)j * N + x] - au;
+            acc += rv * rv;
+            out = (float)rv;
+        }
+        if (x < P) __builtin_nontemporal_store(out, r + (long long)j * P + x);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            a[q] = b[q];
+            b[q] = c[q];
+        }
     }
-    const T au = (T(20) * u[o] - T(4) * (u[o - 1] + u[o - N] + u[o + 1] + u[o + N]) -
-                  (u[o - 1 - N] + u[o - 1 + N] + u[o + 1 - N] + u[o + 1 + N])) * inv6h2;
-    r[o] = (TR)(f[o] - au);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m);
+    if (lane == 0) wsum[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+// part[n] = sum of part[0 .. n) in a fixed order (one workgroup: strided
+// per-thread sums, then a tree over the 256 threads).
+__global__ __launch_bounds__(256)
+void pj_sum_k(int n, double* __restrict__ part)
+{
+    __shared__ double t[256];
+    double s = 0.0;
+    for (int i = (int)threadIdx.x; i < n; i += 256) s += part[i];
+    t[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if ((int)threadIdx.x < m) t[threadIdx.x] += t[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[n] = t[0];
 }
 
 // full weighting: fc(I,J) from rf around (2I, 2J); coarse boundary = 0
@@ -210,17 +266,6 @@ void pj_zero_k(long long n, float* __restrict__ p)
 {
     for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256)
         p[e] = 0.f;
-}
-
-// dense N x N FP64 -> FP32 rows of pitch P (padding 0)
-__global__ __launch_bounds__(256)
-void pj_to_f32_k(int N, int P, const double* __restrict__ s, float* __restrict__ d)
-{
-    const long long n = (long long)N * P;
-    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
-        const long long j = e / P, i = e - j * P;
-        d[e] = i < N ? (float)s[j * N + i] : 0.f;
-    }
 }
 
 // dense N x N FP64 u += FP32 rows of pitch P
@@ -342,11 +387,14 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
             hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(ne)), dim3(256), 0, st, ne, *p);
         }
     }
-    double* r64 = nullptr;
-    if (int rc = alloc((size_t)np * sizeof(double), (void**)&r64)) return fail(rc);
+    // partial sums of r^2 of the fused residual (one per workgroup, then the total)
+    const int res_ntx = (lv[0].P + 255) / 256;
+    const int res_blocks = res_ntx * ((N + RES_RY - 1) / RES_RY);
+    double* part = nullptr;
+    if (int rc = alloc((size_t)(res_blocks + 1) * sizeof(double), (void**)&part)) return fail(rc);
 
     kg_register_kernel("poisson_smooth", (const void*)pj_smooth4_k);
-    double st_f[5], st_r[5];
+    double st_f[5];
     if (int rc = kg_field_stats(ctx, KG_F64, (size_t)np, f, KG_F64, nullptr, st_f)) return fail(rc);
     const double fnorm = std::sqrt(st_f[1]);
     int outer = 0, cycles = 0;
@@ -355,18 +403,21 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
     KG_HIP(hipEventCreate(&e0));
     KG_HIP(hipEventCreate(&e1));
     (void)hipEventRecord(e0, st);
+    Level& F = lv[0];
     for (;;) {
-        hipLaunchKernelGGL((pj_residual_k<double, double>), grid2(N), dim3(PT, PT), 0, st, N,
-                           1.0 / (6.0 * h * h), (const double*)u, f, r64);
-        if (int rc = kg_field_stats(ctx, KG_F64, (size_t)np, r64, KG_F64, nullptr, st_r)) return fail(rc);
-        rel = fnorm > 0 ? std::sqrt(st_r[1]) / fnorm : 0.0;
+        // r = f - A u in FP64: its FP32 copy is the right-hand side of the
+        // inner solve (F.f), its squared norm decides the stop
+        hipLaunchKernelGGL(pj_residual64_k, dim3(res_blocks), dim3(256), 0, st, N, F.P, res_ntx,
+                           1.0 / (6.0 * h * h), (const double*)u, f, F.f, part);
+        hipLaunchKernelGGL(pj_sum_k, dim3(1), dim3(256), 0, st, res_blocks, part);
+        double sumsq = 0.0;
+        KG_HIP(hipMemcpyAsync(&sumsq, part + res_blocks, sizeof(double), hipMemcpyDeviceToHost, st));
+        KG_HIP(hipStreamSynchronize(st));
+        rel = fnorm > 0 ? std::sqrt(sumsq) / fnorm : 0.0;
         if (rep && outer < KG_POISSON_HISTORY) rep->history[outer] = rel;
         if (rel < tol || outer >= max_outer) break;
         // inner FP32 solve of A e = r, e0 = 0
-        Level& F = lv[0];
         const long long nf = (long long)F.N * F.P;
-        hipLaunchKernelGGL(pj_to_f32_k, dim3(blocks1(nf)), dim3(256), 0, st, F.N, F.P,
-                           (const double*)r64, F.f);
         hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nf)), dim3(256), 0, st, nf, F.u);
         hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nf)), dim3(256), 0, st, nf, F.t);
         for (int c = 0; c < vcycles; c++) vcycle(st, lv, 0, pre, post, omega);

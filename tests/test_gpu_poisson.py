@@ -74,6 +74,21 @@ def test_poisson_random_rhs(gpu_ctx, kgmod):
     du.free()
 
 
+def test_poisson_first_residual_norm(gpu_ctx, kgmod):
+    """From u = 0 the first recorded residual is ||f interior||_2 / ||f||_2:
+    the solver's fused residual pass sums r^2 over every interior point and
+    leaves the boundary (where r = 0) out.  f is nonzero on the boundary here,
+    so the ratio is below 1."""
+    n = 512
+    rng = np.random.default_rng(11)
+    f = rng.uniform(-1, 1, (n + 1, n + 1))
+    df, du = gpu_ctx.array(f), gpu_ctx.array(np.zeros_like(f))
+    rep = kgmod.poisson_solve(gpu_ctx, n, df, du, tol=1e-6)
+    want = np.linalg.norm(f[1:-1, 1:-1]) / np.linalg.norm(f)
+    assert want < 0.999
+    assert abs(rep["history"][0] - want) <= 1e-13 * want, (rep["history"][0], want)
+
+
 def test_poisson_rejects_bad_arguments(gpu_ctx, kgmod):
     a = gpu_ctx.empty((9 * 9,), np.float64)
     b = gpu_ctx.empty((9 * 9,), np.float64)
