@@ -128,7 +128,8 @@ void pj_residual4_k(int N, int P, int ntx, float inv6h2, const float* __restrict
 // The outer correction's FP64 residual, fused with its FP32 copy and its
 // squared norm: r = f - (20 u - 4 (u_W + u_S + u_E + u_N) - (u_SW + u_NW +
 // u_SE + u_NE)) / (6 h^2), summed in that order in FP64, written as
-// FP32 rows of pitch P, 0 on the boundary and the padding; the sum of r^2
+// FP32 rows of pitch P, 0 on the boundary and the padding, and e = 0 (the
+// inner solve's initial guess) at the same points; the sum of r^2
 // over the workgroup's points goes to part[blockIdx.x] (lanes, then waves,
 // then the 4 wave sums in order: a fixed order).  A workgroup of 4 waves
 // takes 256 consecutive columns of RES_RY rows; a lane marches its column
@@ -138,7 +139,8 @@ constexpr int RES_RY = 8;
 
 __global__ __launch_bounds__(256)
 void pj_residual64_k(int N, int P, int ntx, double inv6h2, const double* __restrict__ u,
-                     const double* __restrict__ f, float* __restrict__ r, double* __restrict__ part)
+                     const double* __restrict__ f, float* __restrict__ r, float* __restrict__ e,
+                     double* __restrict__ part)
 {
     __shared__ double wsum[4];
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
@@ -167,7 +169,10 @@ void pj_residual64_k(int N, int P, int ntx, double inv6h2, const double* __restr
             acc += rv * rv;
             out = (float)rv;
         }
-        if (x < P) __builtin_nontemporal_store(out, r + (long long)j * P + x);
+        if (x < P) {
+            __builtin_nontemporal_store(out, r + (long long)j * P + x);
+            __builtin_nontemporal_store(0.f, e + (long long)j * P + x);
+        }
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             a[q] = b[q];
@@ -198,13 +203,17 @@ void pj_sum_k(int n, double* __restrict__ part)
     if (threadIdx.x == 0) part[n] = t[0];
 }
 
-// full weighting: fc(I,J) from rf around (2I, 2J); coarse boundary = 0
+// full weighting: fc(I,J) from rf around (2I, 2J); coarse boundary = 0.  Also
+// sets the coarse correction uc to 0 (the initial guess of the coarse solve)
+// at the same points; its padding columns are never written other than 0.
 __global__ __launch_bounds__(PT * PT)
-void pj_restrict_k(int Nc, int Pc, int Pf, const float* __restrict__ rf, float* __restrict__ fc)
+void pj_restrict_k(int Nc, int Pc, int Pf, const float* __restrict__ rf, float* __restrict__ fc,
+                   float* __restrict__ uc)
 {
     const int I = blockIdx.x * PT + threadIdx.x, J = blockIdx.y * PT + threadIdx.y;
     if (I >= Nc || J >= Nc) return;
     const long long oc = (long long)J * Pc + I;
+    uc[oc] = 0.f;
     if (I == 0 || J == 0 || I == Nc - 1 || J == Nc - 1) {
         fc[oc] = 0.f;
         return;
@@ -310,8 +319,8 @@ void smooth(hipStream_t st, Level& L, int n, double omega)
     for (int s = 0; s < n; s++) {
         hipLaunchKernelGGL(pj_smooth4_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx, c0,
                            c1, c2, cf, L.u, L.f, L.t);
-        // the boundary rows of t are 0 like u's (both zeroed at the start of
-        // the cycle, never written), the boundary columns written as 0
+        // the boundary rows of t are 0 like u's (zeroed at allocation, never
+        // written), the boundary columns written as 0
         std::swap(L.u, L.t);
     }
 }
@@ -330,10 +339,9 @@ void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post,
     hipLaunchKernelGGL(pj_residual4_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx,
                        (float)(1.0 / (6.0 * L.h * L.h)), L.u, L.f, L.r);
     Level& C = lv[l + 1];
-    hipLaunchKernelGGL(pj_restrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P, L.r, C.f);
-    const long long nc = (long long)C.N * C.P;
-    hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nc)), dim3(256), 0, st, nc, C.u);
-    hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nc)), dim3(256), 0, st, nc, C.t);
+    // C.u = 0 by the restriction; C.t needs no clearing: every sweep rewrites
+    // its rows 1 .. N-2 whole and its boundary rows stay 0 from allocation
+    hipLaunchKernelGGL(pj_restrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P, L.r, C.f, C.u);
     vcycle(st, lv, l + 1, pre, post, omega);
     {
         const int ntx = (L.P + 255) / 256;
@@ -408,7 +416,7 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
         // r = f - A u in FP64: its FP32 copy is the right-hand side of the
         // inner solve (F.f), its squared norm decides the stop
         hipLaunchKernelGGL(pj_residual64_k, dim3(res_blocks), dim3(256), 0, st, N, F.P, res_ntx,
-                           1.0 / (6.0 * h * h), (const double*)u, f, F.f, part);
+                           1.0 / (6.0 * h * h), (const double*)u, f, F.f, F.u, part);
         hipLaunchKernelGGL(pj_sum_k, dim3(1), dim3(256), 0, st, res_blocks, part);
         double sumsq = 0.0;
         KG_HIP(hipMemcpyAsync(&sumsq, part + res_blocks, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -416,10 +424,7 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
         rel = fnorm > 0 ? std::sqrt(sumsq) / fnorm : 0.0;
         if (rep && outer < KG_POISSON_HISTORY) rep->history[outer] = rel;
         if (rel < tol || outer >= max_outer) break;
-        // inner FP32 solve of A e = r, e0 = 0
-        const long long nf = (long long)F.N * F.P;
-        hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nf)), dim3(256), 0, st, nf, F.u);
-        hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(nf)), dim3(256), 0, st, nf, F.t);
+        // inner FP32 solve of A e = r from e = 0 (F.u, set by the residual pass)
         for (int c = 0; c < vcycles; c++) vcycle(st, lv, 0, pre, post, omega);
         cycles += vcycles;
         hipLaunchKernelGGL(pj_add_k, dim3(blocks1(np)), dim3(256), 0, st, F.N, F.P, (const float*)F.u, u);
