@@ -129,12 +129,12 @@ void pj_residual4_k(int N, int P, int ntx, float inv6h2, const float* __restrict
 // squared norm: r = f - (20 u - 4 (u_W + u_S + u_E + u_N) - (u_SW + u_NW +
 // u_SE + u_NE)) / (6 h^2), summed in that order in FP64, written as
 // FP32 rows of pitch P, 0 on the boundary and the padding, and e = 0 (the
-// inner solve's initial guess) at the same points; the sum of r^2
-// over the workgroup's points goes to part[blockIdx.x] (lanes, then waves,
-// then the 4 wave sums in order: a fixed order).  A workgroup of 4 waves
-// takes 256 consecutive columns of RES_RY rows; a lane marches its column
-// down the rows with the 3 x 3 neighbourhood in registers (one new row of 3
-// values per step, the side values from L1).
+// inner solve's initial guess) at the same points; the sum of r^2 over the
+// workgroup's points goes to part[blockIdx.x] (rows of a lane in order, then
+// lanes and waves by a fixed tree).  A workgroup of 4 waves takes 256
+// consecutive columns of RES_RY rows; a lane loads the RES_RY + 2 rows of
+// its column and both neighbours (side values from L1) and the RES_RY
+// values of f before the first use.
 constexpr int RES_RY = 8;
 
 __global__ __launch_bounds__(256)
@@ -146,37 +146,37 @@ void pj_residual64_k(int N, int P, int ntx, double inv6h2, const double* __restr
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
     const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + w * 64 + lane;
     const int j0 = (int)(blockIdx.x / (unsigned)ntx) * RES_RY;
-    const int j1 = min(j0 + RES_RY, N);
     // clamped columns and rows: they only feed points that are not computed
     const int xc = min(x, N - 1), xl = max(xc - 1, 0), xr = min(xc + 1, N - 1);
-    auto row = [&](int j, double (&v)[3]) {
-        const long long o = (long long)min(max(j, 0), N - 1) * N;
-        v[0] = u[o + xl];
-        v[1] = u[o + xc];
-        v[2] = u[o + xr];
-    };
-    double a[3], b[3], c[3];   // rows j-1, j, j+1 at columns x-1, x, x+1
-    row(j0 - 1, a);
-    row(j0, b);
+    double v[RES_RY + 2][3], fv[RES_RY];
+#pragma unroll
+    for (int q = 0; q < RES_RY + 2; q++) {
+        const long long o = (long long)min(max(j0 - 1 + q, 0), N - 1) * N;
+        v[q][0] = u[o + xl];
+        v[q][1] = u[o + xc];
+        v[q][2] = u[o + xr];
+    }
+#pragma unroll
+    for (int q = 0; q < RES_RY; q++) fv[q] = f[(long long)min(j0 + q, N - 1) * N + xc];
     double acc = 0.0;
-    for (int j = j0; j < j1; j++) {
-        row(j + 1, c);
+#pragma unroll
+    for (int q = 0; q < RES_RY; q++) {
+        const int j = j0 + q;
+        if (j >= N) break;
         float out = 0.f;
         if (x >= 1 && x < N - 1 && j >= 1 && j < N - 1) {
+            const double(&a)[3] = v[q];
+            const double(&b)[3] = v[q + 1];
+            const double(&c)[3] = v[q + 2];
             const double au = (20.0 * b[1] - 4.0 * (b[0] + a[1] + b[2] + c[1]) -
                                (a[0] + c[0] + a[2] + c[2])) * inv6h2;
-            const double rv = f[(long long)j * N + x] - au;
+            const double rv = fv[q] - au;
             acc += rv * rv;
             out = (float)rv;
         }
         if (x < P) {
             __builtin_nontemporal_store(out, r + (long long)j * P + x);
             __builtin_nontemporal_store(0.f, e + (long long)j * P + x);
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            a[q] = b[q];
-            b[q] = c[q];
         }
     }
 #pragma unroll
@@ -186,17 +186,17 @@ void pj_residual64_k(int N, int P, int ntx, double inv6h2, const double* __restr
     if (threadIdx.x == 0) part[blockIdx.x] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
 }
 
-// part[n] = sum of part[0 .. n) in a fixed order (one workgroup: strided
-// per-thread sums, then a tree over the 256 threads).
-__global__ __launch_bounds__(256)
+// part[n] = sum of part[0 .. n) in a fixed order (one workgroup of 1024
+// threads: strided per-thread sums, then a tree).
+__global__ __launch_bounds__(1024)
 void pj_sum_k(int n, double* __restrict__ part)
 {
-    __shared__ double t[256];
+    __shared__ double t[1024];
     double s = 0.0;
-    for (int i = (int)threadIdx.x; i < n; i += 256) s += part[i];
+    for (int i = (int)threadIdx.x; i < n; i += 1024) s += part[i];
     t[threadIdx.x] = s;
     __syncthreads();
-    for (int m = 128; m >= 1; m >>= 1) {
+    for (int m = 512; m >= 1; m >>= 1) {
         if ((int)threadIdx.x < m) t[threadIdx.x] += t[threadIdx.x + m];
         __syncthreads();
     }
@@ -417,7 +417,7 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
         // inner solve (F.f), its squared norm decides the stop
         hipLaunchKernelGGL(pj_residual64_k, dim3(res_blocks), dim3(256), 0, st, N, F.P, res_ntx,
                            1.0 / (6.0 * h * h), (const double*)u, f, F.f, F.u, part);
-        hipLaunchKernelGGL(pj_sum_k, dim3(1), dim3(256), 0, st, res_blocks, part);
+        hipLaunchKernelGGL(pj_sum_k, dim3(1), dim3(1024), 0, st, res_blocks, part);
         double sumsq = 0.0;
         KG_HIP(hipMemcpyAsync(&sumsq, part + res_blocks, sizeof(double), hipMemcpyDeviceToHost, st));
         KG_HIP(hipStreamSynchronize(st));
