@@ -126,9 +126,9 @@ void life_u8_cells_k(int nx, int ny, int ntx, const uint8_t* __restrict__ c0,
 // nx % 16 == 0 and 1 <= nx <= 2^24 (offsets within a band fit 31 bits).
 // Bands of 4 rows measured fastest at 4096^2 .. 16384^2 (16-row bands 15 %
 // slower).  Tile order KG_ORD_LIFE8: chunks of 8 bands per XCD, so a band's
-// neighbour rows and segments share its L2 (8192^2, 20-launch means: 25.2 us;
-// hardware order 29.0, XCD-contiguous 25.6, chunks of 2 or 4 bands 27.4 and
-// 26.3).
+// neighbour rows and segments share its L2 (ping-pong steps, 50-launch means:
+// 8192^2 30.7 us, 16384^2 100 us; hardware order 31.4 / 107, XCD-contiguous
+// 37.9 / 108, chunks of 2, 4 or 16 bands within 2 %).
 __global__ __launch_bounds__(64 * LIFE_WAVES)
 void life_u8_k(int nx, int ny, int nsx, const uint8_t* __restrict__ c0, uint8_t* __restrict__ c1)
 {
