@@ -94,6 +94,80 @@ void pj_smooth4_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
     }
 }
 
+// Two damped Jacobi sweeps in one pass, bit-identical to two pj_smooth4_k
+// launches (same expression and order, no contraction): the lane forms the
+// first sweep's values on the 6 x 6 block around its 4 points x PY rows
+// (columns x-1 .. x+4, rows j0-1 .. j0+PY; 0 outside rows/columns 1 .. N-2,
+// as the first launch would have written them), then the second sweep on
+// its own points.  u rows j0-2 .. j0+PY+1 and f rows j0-1 .. j0+PY come
+// through the caches; HBM sees each array once per two sweeps.
+__global__ __launch_bounds__(64 * VW)
+void pj_smooth4x2_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
+                    const float* __restrict__ u, const float* __restrict__ f,
+                    float* __restrict__ uo)
+{
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + lane * 4;
+    const int j0 = 1 + ((int)(blockIdx.x / (unsigned)ntx) * VW + w) * PY;
+    if (x >= P || j0 >= N - 1) return;
+    const int xl = x >= 4 ? x - 4 : x, xr = x + 4 < P ? x + 4 : x;
+    // uu[q][c]: u at row j0-2+q, column x-2+c (c = 0 .. 7)
+    float uu[PY + 4][8];
+#pragma unroll
+    for (int q = 0; q < PY + 4; q++) {
+        const float* row = u + (long long)min(max(j0 - 2 + q, 0), N - 1) * P;
+        const f4 a = ldf4(row + xl), b = ldf4(row + x), c = ldf4(row + xr);
+        uu[q][0] = a[2];
+        uu[q][1] = a[3];
+#pragma unroll
+        for (int v = 0; v < 4; v++) uu[q][2 + v] = b[v];
+        uu[q][6] = c[0];
+        uu[q][7] = c[1];
+    }
+    // ff[q][c]: f at row j0-1+q, column x-1+c (c = 0 .. 5)
+    float ff[PY + 2][6];
+#pragma unroll
+    for (int q = 0; q < PY + 2; q++) {
+        const float* row = f + (long long)min(max(j0 - 1 + q, 0), N - 1) * P;
+        const f4 a = ldf4(row + xl), b = ldf4(row + x), c = ldf4(row + xr);
+        ff[q][0] = a[3];
+#pragma unroll
+        for (int v = 0; v < 4; v++) ff[q][1 + v] = b[v];
+        ff[q][5] = c[0];
+    }
+    // first sweep: s1[q][c] at row j0-1+q, column x-1+c
+    float s1[PY + 2][6];
+#pragma unroll
+    for (int q = 0; q < PY + 2; q++) {
+        const int j = j0 - 1 + q;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            const int xc = x - 1 + c;
+#define U(dx, dy) uu[q + 1 + (dy)][c + 1 + (dx)]
+            const float val = c0 * U(0, 0) + c1 * (U(-1, 0) + U(0, -1) + U(1, 0) + U(0, 1)) +
+                              c2 * (U(-1, -1) + U(-1, 1) + U(1, -1) + U(1, 1)) + cf * ff[q][c];
+#undef U
+            s1[q][c] = (xc >= 1 && xc < N - 1 && j >= 1 && j < N - 1) ? val : 0.f;
+        }
+    }
+    // second sweep on the lane's points
+#pragma unroll
+    for (int r = 0; r < PY; r++) {
+        const int j = j0 + r;
+        if (j >= N - 1) break;
+        f4 o;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+#define U(dx, dy) s1[r + 1 + (dy)][v + 1 + (dx)]
+            const float val = c0 * U(0, 0) + c1 * (U(-1, 0) + U(0, -1) + U(1, 0) + U(0, 1)) +
+                              c2 * (U(-1, -1) + U(-1, 1) + U(1, -1) + U(1, 1)) + cf * ff[r + 1][v + 1];
+#undef U
+            o[v] = (x + v >= 1 && x + v < N - 1) ? val : 0.f;
+        }
+        __builtin_nontemporal_store(o, (__attribute__((address_space(1))) f4*)(uo + (long long)j * P + x));
+    }
+}
+
 // r = f - A u (FP32) on 4-point pieces, same expression as pj_residual64_k
 // (the FP64 residual of the dense fine grid); writes rows 1 .. N-2 whole (0
 // at the boundary and padding)
@@ -317,6 +391,13 @@ void smooth(hipStream_t st, Level& L, int n, double omega)
     const float cf = (float)(3 * omega * L.h * L.h / 10);
     const Grid4 g = grid4(L);
     for (int s = 0; s < n; s++) {
+        if (s + 1 < n) {   // two sweeps in one pass
+            hipLaunchKernelGGL(pj_smooth4x2_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx,
+                               c0, c1, c2, cf, L.u, L.f, L.t);
+            std::swap(L.u, L.t);
+            s++;
+            continue;
+        }
         hipLaunchKernelGGL(pj_smooth4_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx, c0,
                            c1, c2, cf, L.u, L.f, L.t);
         // the boundary rows of t are 0 like u's (zeroed at allocation, never
