@@ -101,6 +101,10 @@ void pj_smooth4_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
 // as the first launch would have written them), then the second sweep on
 // its own points.  u rows j0-2 .. j0+PY+1 and f rows j0-1 .. j0+PY come
 // through the caches; HBM sees each array once per two sweeps.
+#ifndef KG_SM2_PY
+#define KG_SM2_PY 2   // rows per lane (8192^2 solve: 19.0 ms; 3 rows 19.1, 4 rows 19.4, 6 rows 19.0)
+#endif
+template <int PY>
 __global__ __launch_bounds__(64 * VW)
 void pj_smooth4x2_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
                     const float* __restrict__ u, const float* __restrict__ f,
@@ -392,7 +396,9 @@ void smooth(hipStream_t st, Level& L, int n, double omega)
     const Grid4 g = grid4(L);
     for (int s = 0; s < n; s++) {
         if (s + 1 < n) {   // two sweeps in one pass
-            hipLaunchKernelGGL(pj_smooth4x2_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx,
+            constexpr int PY2 = KG_SM2_PY;
+            const unsigned b2 = (unsigned)g.ntx * (unsigned)((L.N - 2 + VW * PY2 - 1) / (VW * PY2));
+            hipLaunchKernelGGL(pj_smooth4x2_k<PY2>, dim3(b2), dim3(64 * VW), 0, st, L.N, L.P, g.ntx,
                                c0, c1, c2, cf, L.u, L.f, L.t);
             std::swap(L.u, L.t);
             s++;
