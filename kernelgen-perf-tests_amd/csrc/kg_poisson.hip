@@ -96,8 +96,9 @@ void pj_smooth4_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
 
 // Two damped Jacobi sweeps in one pass, bit-identical to two pj_smooth4_k
 // launches (same expression and order, no contraction): the lane forms the
-// first sweep's values on the 6 x 6 block around its 4 points x PY rows
-// (columns x-1 .. x+4, rows j0-1 .. j0+PY; 0 outside rows/columns 1 .. N-2,
+// first sweep's values on the block one point wider on every side than its
+// 4 points x PY rows (columns x-1 .. x+4, rows j0-1 .. j0+PY; 0 outside
+// rows/columns 1 .. N-2,
 // as the first launch would have written them), then the second sweep on
 // its own points.  u rows j0-2 .. j0+PY+1 and f rows j0-1 .. j0+PY come
 // through the caches; HBM sees each array once per two sweeps.
