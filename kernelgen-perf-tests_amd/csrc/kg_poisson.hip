@@ -19,7 +19,7 @@
 // needs only a few digits per correction; FP64 residuals recover the rest.
 //
 // The FP32 levels are stored with rows padded to P = N rounded up to 4
-// floats, so that the fine-level sweeps and residuals work on 16-byte pieces:
+// floats, so that the sweeps and the prolongation work on 16-byte pieces:
 // a lane owns 4 points of PY rows, loads the rows it needs (pieces at x-4, x,
 // x+4) through the caches, and writes whole rows -- interior values, and 0 at
 // the boundary and padding points, which those arrays hold anyway -- so that
@@ -173,37 +173,6 @@ void pj_smooth4x2_k(int N, int P, int ntx, float c0, float c1, float c2, float c
     }
 }
 
-// r = f - A u (FP32) on 4-point pieces, same expression as pj_residual64_k
-// (the FP64 residual of the dense fine grid); writes rows 1 .. N-2 whole (0
-// at the boundary and padding)
-__global__ __launch_bounds__(64 * VW)
-void pj_residual4_k(int N, int P, int ntx, float inv6h2, const float* __restrict__ u,
-                    const float* __restrict__ f, float* __restrict__ r)
-{
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    const int x = (int)(blockIdx.x % (unsigned)ntx) * 256 + lane * 4;
-    const int j0 = 1 + ((int)(blockIdx.x / (unsigned)ntx) * VW + w) * PY;
-    if (x >= P || j0 >= N - 1) return;
-    Win4 W;
-    load_win4(u, N, P, x, j0, W);
-#pragma unroll
-    for (int q = 0; q < PY; q++) {
-        const int j = j0 + q;
-        if (j >= N - 1) break;
-        const f4 fv = ldf4(f + (long long)j * P + x);
-        f4 o;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-#define U(dx, dy) W.w[q + 1 + (dy)][4 + v + (dx)]
-            const float au = (20.f * U(0, 0) - 4.f * (U(-1, 0) + U(0, -1) + U(1, 0) + U(0, 1)) -
-                              (U(-1, -1) + U(-1, 1) + U(1, -1) + U(1, 1))) * inv6h2;
-#undef U
-            o[v] = (x + v >= 1 && x + v < N - 1) ? fv[v] - au : 0.f;
-        }
-        __builtin_nontemporal_store(o, (__attribute__((address_space(1))) f4*)(r + (long long)j * P + x));
-    }
-}
-
 // The outer correction's FP64 residual, fused with its FP32 copy and its
 // squared norm: r = f - (20 u - 4 (u_W + u_S + u_E + u_N) - (u_SW + u_NW +
 // u_SE + u_NE)) / (6 h^2), summed in that order in FP64, written as
@@ -282,12 +251,16 @@ void pj_sum_k(int n, double* __restrict__ part)
     if (threadIdx.x == 0) part[n] = t[0];
 }
 
-// full weighting: fc(I,J) from rf around (2I, 2J); coarse boundary = 0.  Also
-// sets the coarse correction uc to 0 (the initial guess of the coarse solve)
-// at the same points; its padding columns are never written other than 0.
+// Residual and full-weighting restriction in one pass (FP32): the coarse
+// right-hand side
+//   fc(I,J) = (4 r_C + 2 (r_W + r_E + r_S + r_N) + (r_SW + r_SE + r_NW + r_NE)) / 16
+// of the fine residual r = f - (20 u - 4 (u_W + u_S + u_E + u_N) - (u_SW +
+// u_NW + u_SE + u_NE)) / (6 h^2) at the 3 x 3 fine points around (2I, 2J),
+// summed in those orders; the fine residual is never stored.  Coarse
+// boundary 0; uc = 0 at the same points (the coarse solve's initial guess).
 __global__ __launch_bounds__(PT * PT)
-void pj_restrict_k(int Nc, int Pc, int Pf, const float* __restrict__ rf, float* __restrict__ fc,
-                   float* __restrict__ uc)
+void pj_resrestrict_k(int Nc, int Pc, int Pf, float inv6h2, const float* __restrict__ u,
+                      const float* __restrict__ f, float* __restrict__ fc, float* __restrict__ uc)
 {
     const int I = blockIdx.x * PT + threadIdx.x, J = blockIdx.y * PT + threadIdx.y;
     if (I >= Nc || J >= Nc) return;
@@ -297,10 +270,28 @@ void pj_restrict_k(int Nc, int Pc, int Pf, const float* __restrict__ rf, float* 
         fc[oc] = 0.f;
         return;
     }
-    const int Nf = Pf;   // row pitch of the fine level
-    const long long o = (long long)(2 * J) * Nf + 2 * I;
-    fc[oc] = (4.f * rf[o] + 2.f * (rf[o - 1] + rf[o + 1] + rf[o - Nf] + rf[o + Nf]) +
-              (rf[o - 1 - Nf] + rf[o + 1 - Nf] + rf[o - 1 + Nf] + rf[o + 1 + Nf])) * 0.0625f;
+    // fine rows 2J-2 .. 2J+2 and columns 2I-2 .. 2I+2 lie in 0 .. Nf-1
+    const long long o = (long long)(2 * J) * Pf + 2 * I;
+    float uu[5][5];
+#pragma unroll
+    for (int dy = -2; dy <= 2; dy++)
+#pragma unroll
+        for (int dx = -2; dx <= 2; dx++) uu[dy + 2][dx + 2] = u[o + (long long)dy * Pf + dx];
+    float rr[3][3];
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+        for (int dx = -1; dx <= 1; dx++) {
+#define U(a, b) uu[dy + 2 + (b)][dx + 2 + (a)]
+            const float au = (20.f * U(0, 0) - 4.f * (U(-1, 0) + U(0, -1) + U(1, 0) + U(0, 1)) -
+                              (U(-1, -1) + U(-1, 1) + U(1, -1) + U(1, 1))) * inv6h2;
+#undef U
+            rr[dy + 1][dx + 1] = f[o + (long long)dy * Pf + dx] - au;
+        }
+#define R(a, b) rr[(b) + 1][(a) + 1]
+    fc[oc] = (4.f * R(0, 0) + 2.f * (R(-1, 0) + R(1, 0) + R(0, -1) + R(0, 1)) +
+              (R(-1, -1) + R(1, -1) + R(-1, 1) + R(1, 1))) * 0.0625f;
+#undef R
 }
 
 // u_f += bilinear interpolation of e_c on 4-point pieces of fine rows
@@ -370,7 +361,7 @@ void pj_add_k(int N, int P, const float* __restrict__ e, double* __restrict__ u)
 struct Level {
     int N, P;   // vertices per side, row pitch (N rounded up to 4)
     double h;
-    float *u, *t, *f, *r;
+    float *u, *t, *f;
 };
 
 dim3 grid2(int N) { return dim3((N + PT - 1) / PT, (N + PT - 1) / PT); }
@@ -423,13 +414,12 @@ void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post,
         return;
     }
     smooth(st, L, pre, omega);
-    const Grid4 g = grid4(L);
-    hipLaunchKernelGGL(pj_residual4_k, dim3(g.blocks), dim3(64 * VW), 0, st, L.N, L.P, g.ntx,
-                       (float)(1.0 / (6.0 * L.h * L.h)), L.u, L.f, L.r);
     Level& C = lv[l + 1];
-    // C.u = 0 by the restriction; C.t needs no clearing: every sweep rewrites
-    // its rows 1 .. N-2 whole and its boundary rows stay 0 from allocation
-    hipLaunchKernelGGL(pj_restrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P, L.r, C.f, C.u);
+    // C.f = restricted residual, C.u = 0; C.t needs no clearing: every sweep
+    // rewrites its rows 1 .. N-2 whole and its boundary rows stay 0 from
+    // allocation
+hipLaunchKernelGGL(pj_resrestrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P,
+                       (float)(1.0 / (6.0 * L.h * L.h)), L.u, L.f, C.f, C.u);
     vcycle(st, lv, l + 1, pre, post, omega);
     {
         const int ntx = (L.P + 255) / 256;
@@ -460,7 +450,7 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
     const int pre = 2, post = 2;
     std::vector<Level> lv;
     for (int m = n; m >= 2; m /= 2)
-        lv.push_back(Level{m + 1, (m + 1 + 3) / 4 * 4, 1.0 / m, nullptr, nullptr, nullptr, nullptr});
+        lv.push_back(Level{m + 1, (m + 1 + 3) / 4 * 4, 1.0 / m, nullptr, nullptr, nullptr});
     std::vector<void*> allocs;
     auto fail = [&](int rc) {
         (void)hipStreamSynchronize(st);
@@ -478,7 +468,7 @@ extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, 
     };
     for (auto& L : lv) {
         const long long ne = (long long)L.N * L.P;
-        for (float** p : {&L.u, &L.t, &L.f, &L.r}) {
+        for (float** p : {&L.u, &L.t, &L.f}) {
             if (int rc = alloc((size_t)ne * sizeof(float), (void**)p)) return fail(rc);
             hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(ne)), dim3(256), 0, st, ne, *p);
         }
