@@ -418,7 +418,7 @@ void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post,
     // C.f = restricted residual, C.u = 0; C.t needs no clearing: every sweep
     // rewrites its rows 1 .. N-2 whole and its boundary rows stay 0 from
     // allocation
-hipLaunchKernelGGL(pj_resrestrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P,
+    hipLaunchKernelGGL(pj_resrestrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P,
                        (float)(1.0 / (6.0 * L.h * L.h)), L.u, L.f, C.f, C.u);
     vcycle(st, lv, l + 1, pre, post, omega);
     {
