@@ -262,6 +262,34 @@ extern "C" int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, 
 #ifndef KG_WF32_KC
 #define KG_WF32_KC 64
 #endif
+// BF16 shapes measured at 512^3 with the hardware bf16 conversion:
+// laplacian 16-byte lanes, W=8/D=2/KC=8 110 us (KC=16 113, KC=4 123, W=12
+// 128); wave13pt 16-byte lanes, W=12/D=2/KC=64 200 us (KC=32 204-221, D=3
+// 215; 8-byte lanes W=16/KC=32 232-244).
+#ifndef KG_LBF_V
+#define KG_LBF_V 8
+#endif
+#ifndef KG_LBF_W
+#define KG_LBF_W 8
+#endif
+#ifndef KG_LBF_D
+#define KG_LBF_D 2
+#endif
+#ifndef KG_LBF_KC
+#define KG_LBF_KC 8
+#endif
+#ifndef KG_WBF_V
+#define KG_WBF_V 8
+#endif
+#ifndef KG_WBF_W
+#define KG_WBF_W 12
+#endif
+#ifndef KG_WBF_D
+#define KG_WBF_D 2
+#endif
+#ifndef KG_WBF_KC
+#define KG_WBF_KC 64
+#endif
 
 extern "C" int kg_laplacian_f32(kg_ctx* ctx, int nx, int ny, int ns, float alpha, float beta,
                                 const float* w0, float* w1)
@@ -282,7 +310,7 @@ extern "C" int kg_laplacian_bf16(kg_ctx* ctx, int nx, int ny, int ns, float alph
 {
     KG_REQUIRE(w0 && w1, "kg_laplacian_bf16: null buffer");
     KG_REQUIRE((const void*)w0 != (const void*)w1, "kg_laplacian_bf16: w0 and w1 must be distinct");
-    return launch_typed<LaplacianOp, kgp::BF16, 8, 8, 2, 16>(ctx, "laplacian_bf16", nx, ny, ns,
+    return launch_typed<LaplacianOp, kgp::BF16, KG_LBF_V, KG_LBF_W, KG_LBF_D, KG_LBF_KC>(ctx, "laplacian_bf16", nx, ny, ns,
                                                              kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
@@ -314,7 +342,7 @@ extern "C" int kg_wave13pt_bf16(kg_ctx* ctx, int nx, int ny, int ns, float m0, f
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_bf16: w2 aliases an input");
     // 8-byte lanes: with 16-byte lanes (8 points) the 1024-thread tile needs
     // more than the 128 registers a lane may hold at that size and spills.
-    return launch_typed<Wave13ptOp, kgp::BF16, 4, 16, 2, 32>(ctx, "wave13pt_bf16", nx, ny, ns,
+    return launch_typed<Wave13ptOp, kgp::BF16, KG_WBF_V, KG_WBF_W, KG_WBF_D, KG_WBF_KC>(ctx, "wave13pt_bf16", nx, ny, ns,
                                                              kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
