@@ -241,9 +241,10 @@ extern "C" int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, 
 
 // ======================================================= precision variants
 // Tile shapes (waves, prefetch planes, planes per workgroup) of the FP32
-// kernels, from sweeps at 512^3 (cold caches): laplacian 12/3/16 220 us
-// (8/2/16 227, 8/3/8 221, 16/2/16 228); wave13pt 16/3/64 326 us (16/2/32
-// 343, 12/3/32 358).
+// kernels, from sweeps at 512^3 (cold caches) in chunked tile order:
+// laplacian 12/3/8 201 us (12/3/16 209, 12/2/16 210, 8/3/16 214, 16/3/16
+// 227, 12/3/32 221); wave13pt 16/3/64 312 us (12/3/64 309, 16/2/64 315,
+// 16/3/32 320).
 #ifndef KG_LF32_W
 #define KG_LF32_W 12
 #endif
@@ -251,7 +252,7 @@ extern "C" int kg_convert(kg_ctx* ctx, int dst_dtype, void* dst, int src_dtype, 
 #define KG_LF32_D 3
 #endif
 #ifndef KG_LF32_KC
-#define KG_LF32_KC 16
+#define KG_LF32_KC 8
 #endif
 #ifndef KG_WF32_W
 #define KG_WF32_W 16
