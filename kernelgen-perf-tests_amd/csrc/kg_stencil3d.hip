@@ -354,10 +354,16 @@ struct LapgsrbOp {
 #define KG_LAP_KC 8
 #endif
 template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0, KG_LAP_KC>;
-// gradient: the laplacian's first shape (W=8/D=2/KC=8) in chunks of 2 tile
-// rows, at most 2 workgroups (16 waves) per CU: 201 us (3 workgroups 208;
-// W=12 212, W=16 216, KC=16 224, D=3 209, XCD-contiguous 219).
-template <int V> using GradCfg = kgp::Cfg<V, 8, 1, 0, 0, 1, 1, 2, 0, 8>;
+// gradient: W=12/D=2/KC=8 in chunks of 2 tile rows, one workgroup (12 waves)
+// per CU: 196 us (W=8 with 2 workgroups 201, 3 workgroups 208; W=12 with 2
+// 213; W=16 with 1 214; KC=16 224, D=3 209, XCD-contiguous 219).
+#ifndef KG_GRAD_W
+#define KG_GRAD_W 12
+#endif
+#ifndef KG_GRAD_OCC
+#define KG_GRAD_OCC 1
+#endif
+template <int V> using GradCfg = kgp::Cfg<V, KG_GRAD_W, 1, 0, 0, 1, 1, 2, 0, 8>;
 template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #ifndef KG_GSRB_W
 #define KG_GSRB_W 12
@@ -477,7 +483,7 @@ extern "C" int kg_gradient(kg_ctx* ctx, int nx, int ny, int ns, double alpha, do
     KG_REQUIRE(u && ux && uy && uz, "kg_gradient: null buffer");
     KG_REQUIRE(u != ux && u != uy && u != uz && ux != uy && ux != uz && uy != uz,
                "kg_gradient: buffers must be distinct");
-    return launch_plane<GradCfg, GradientOp, 1, 1, KG_ORD_GRAD, 0, 2>(ctx, "gradient", nx, ny, ns,
+    return launch_plane<GradCfg, GradientOp, 1, 1, KG_ORD_GRAD, 0, KG_GRAD_OCC>(ctx, "gradient", nx, ny, ns,
                                                          kg_aligned16(u, ux, uy, uz), [&](auto& op) {
         op.src = u;
         op.ux = ux;
