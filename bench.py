@@ -304,8 +304,8 @@ def main() -> int:
             if k not in keep:
                 del STENCILS[k]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; the stencil path runs on a single "
-              "device (multi-GPU z-slab sharding is described in DESIGN.md, not built)",
+        print("bench.py measures one GPU: the stencil path runs on a single device "
+              "(multi-GPU z-slab sharding is not built)",
               file=sys.stderr)
         return 2
     rng = np.random.default_rng(2024)

@@ -467,8 +467,9 @@ DEF_GAUSS3(f32, float)
 
 /* ---- the suite's single-precision tests (real = float) ------------------ */
 
-/* vecadd: w2 = w0 + w1 (the reference's vecadd source is not in its tree;
- * SURVEY-level description of the test). */
+/* vecadd: w2 = w0 + w1 (the reference's vecadd source is not in its tree:
+ * vecadd/<target>/makefile builds ../vecadd.c, which is absent; this is the
+ * operation the test's name states). */
 void oracle_vecadd_f32(size_t n, const float* w0, const float* w1, float* w2)
 {
     for (size_t i = 0; i < n; i++) w2[i] = w0[i] + w1[i];
