@@ -74,6 +74,31 @@ def test_poisson_random_rhs(gpu_ctx, kgmod):
     du.free()
 
 
+def test_poisson_random_rhs_8192(gpu_ctx, kgmod):
+    """The full-size configuration: 8192^2 cells (8193^2 vertices), random
+    right-hand side in [-1, 1) with a zero boundary, to a relative residual
+    of 1e-10.  The residual is recomputed here in FP64 numpy from the
+    downloaded solution, independently of the solver's own norm."""
+    n = 8192
+    rng = np.random.default_rng(8192)
+    f = rng.uniform(-1, 1, (n + 1, n + 1))
+    f[0, :] = f[-1, :] = f[:, 0] = f[:, -1] = 0
+    df, du = gpu_ctx.array(f), gpu_ctx.array(np.zeros_like(f))
+    rep = kgmod.poisson_solve(gpu_ctx, n, df, du, tol=1e-10)
+    u = du.download()
+    df.free()
+    du.free()
+    r = f - apply_A(u, 1.0 / n)
+    rel = np.linalg.norm(r) / np.linalg.norm(f)
+    assert rel < 1e-10, (rel, rep)
+    # the solver's norm and the independent one agree up to the rounding of
+    # the two FP64 evaluations (different summation orders)
+    assert abs(rel - rep["rel_residual"]) <= 0.1 * rel + 1e-12, (rel, rep["rel_residual"])
+    hist = rep["history"]
+    assert all(b < a for a, b in zip(hist, hist[1:])), hist
+    assert rep["outer"] <= 8 and rep["levels"] == 13, rep
+
+
 def test_poisson_first_residual_norm(gpu_ctx, kgmod):
     """From u = 0 the first recorded residual is ||f interior||_2 / ||f||_2:
     the solver's fused residual pass sums r^2 over every interior point and

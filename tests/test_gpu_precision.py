@@ -87,7 +87,7 @@ def test_typed_matches_oracle(gpu_ctx, oracle, which, dtype):
 
 
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("wave13pt", "f32"),
-                                         ("laplacian", "bf16")])
+                                         ("laplacian", "bf16"), ("wave13pt", "bf16")])
 def test_typed_tile_order_tail(gpu_ctx, oracle, which, dtype):
     """600x300x40: 450 (FP32 laplacian), 500 (BF16 laplacian) and 75
     (wave13pt) workgroups, not whole rounds of 8 chunks of two tile rows; the
@@ -102,8 +102,8 @@ def test_typed_tile_order_tail(gpu_ctx, oracle, which, dtype):
 
 
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("wave13pt", "f32"),
-                                         ("laplacian", "bf16"), ("gaussblur3", "f32"),
-                                         ("gaussblur3", "f64")])
+                                         ("laplacian", "bf16"), ("wave13pt", "bf16"),
+                                         ("gaussblur3", "f32"), ("gaussblur3", "f64")])
 def test_typed_full_size_one_step(gpu_ctx, oracle, which, dtype):
     """512x256x256, one step, bit-exact against the oracle."""
     dims = (512, 256, 256)
@@ -113,6 +113,94 @@ def test_typed_full_size_one_step(gpu_ctx, oracle, which, dtype):
     got = _run_gpu(gpu_ctx, fn, dims, coefs, ins, out, 1, rot)
     for g, w in zip(got, want):
         assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
+
+
+@pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("laplacian", "bf16"),
+                                         ("wave13pt", "f32"), ("wave13pt", "bf16")])
+def test_typed_z_chunk_seams(gpu_ctx, oracle, which, dtype):
+    """264x20x150: more z-planes than one chunk of any typed tile shape (the
+    longest is 64 planes), so chunks start inside the grid and their first
+    planes come from the previous chunk's range; two steps, bit for bit."""
+    dims = (264, 20, 150)
+    fn = f"{which}_{dtype}"
+    ins, out, coefs, rot = _case(which, dtype, dims, 23)
+    want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 2, rot, f32)
+    got = _run_gpu(gpu_ctx, fn, dims, coefs, ins, out, 2, rot)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
+
+
+def _oracle_to_bf16(oracle):
+    fn = oracle.lib.oracle_f32_to_bf16
+    fn.argtypes, fn.restype = [f32], C.c_uint16
+    return fn
+
+
+def _special_f32_patterns():
+    """Every upper half-word (sign, exponent, top 7 mantissa bits) with lower
+    halves that round down, tie (to even either way), round up and carry:
+    NaN payloads (quiet and signalling), +-inf, FLT_MAX, subnormals, +-0."""
+    hi = np.arange(1 << 16, dtype=np.uint32) << 16
+    lo = np.array([0x0000, 0x0001, 0x7FFF, 0x8000, 0x8001, 0xC000, 0xFFFF], np.uint32)
+    return (hi[:, None] | lo[None, :]).ravel()
+
+
+def test_bf16_conversion_special_values(gpu_ctx, kgmod, oracle):
+    """kg_convert F32 -> BF16 runs the conversion the BF16 stencils store
+    through; its bits equal the oracle's round-to-nearest-even rule on
+    458752 patterns that cover ties, NaN payloads, overflow to infinity and
+    subnormals."""
+    to16 = _oracle_to_bf16(oracle)
+    pat = _special_f32_patterns()
+    x = pat.view(np.float32)
+    want = np.array([to16(C.c_float(v)) for v in x.tolist()], np.uint16)
+    dx = gpu_ctx.array(x)
+    dy = gpu_ctx.empty(x.shape, np.uint16)
+    gpu_ctx.call("convert", kgmod.BF16, dy, kgmod.F32, dx, x.size)
+    got = dy.download()
+    dx.free()
+    dy.free()
+    nan_w = (want & 0x7FFF) > 0x7F80
+    nan_g = (got & 0x7FFF) > 0x7F80
+    assert np.array_equal(nan_w, nan_g), "NaN-ness differs"
+    bad = np.flatnonzero((got != want) & ~nan_w)
+    assert bad.size == 0, [(hex(pat[i]), hex(got[i]), hex(want[i])) for i in bad[:8]]
+    # NaNs stay NaNs with the sign kept (payloads are not part of the contract)
+    assert np.array_equal(got[nan_w] >> 15, want[nan_w] >> 15)
+
+
+def test_bf16_stencil_rounding(gpu_ctx, oracle):
+    """laplacian_bf16 with alpha = 1 + 2^-8, beta = 0: every product of a
+    bfloat16 input with alpha has its lowest bits at the rounding position,
+    so the stores round ties to even, carry into the exponent, overflow to
+    infinity (inputs near the bfloat16 maximum) and produce subnormals (tiny
+    inputs); bit-exact against the oracle.  Inputs near the maximum sit on a
+    lattice of pitch 4 in every axis, so no neighbour sum overflows (beta = 0
+    would turn an infinite sum into NaN)."""
+    rng = np.random.default_rng(5)
+    nx, ny, ns = 68, 12, 9
+    n = nx * ny * ns
+    w0 = rng.integers(0, 1 << 16, n, dtype=np.uint16)
+    # magnitudes below 2^97: exponent field at most 0xE0
+    exp = (w0 >> 7) & 0xFF
+    w0 = np.where(exp > 0xE0, w0 & 0x807F | (0xE0 << 7), w0).astype(np.uint16)
+    small = np.array([0x0001, 0x8001, 0x007F, 0x0080, 0x00FF, 0x3F80, 0x3FFF, 0x0000, 0x8000],
+                     np.uint16)
+    idx = rng.choice(n, 600, replace=False)
+    w0[idx] = small[rng.integers(0, small.size, idx.size)]
+    large = np.array([0x7F7F, 0xFF7F, 0x7F7E, 0x7F00], np.uint16)
+    g = w0.reshape(ns, ny, nx)
+    g[::4, ::4, ::4] = large[rng.integers(0, large.size, g[::4, ::4, ::4].shape)]
+    w0 = g.ravel().copy()
+    out = np.zeros(n, np.uint16)
+    coefs = (1.00390625, 0.0)
+    rot = lambda b: [b[1], b[0]]
+    dims = (nx, ny, ns)
+    want = _run_oracle(oracle.lib, "laplacian_bf16", dims, coefs, [w0], out, 1, rot, f32)
+    got = _run_gpu(gpu_ctx, "laplacian_bf16", dims, coefs, [w0], out, 1, rot)
+    assert np.any(want[0] == 0x7F80) and np.any((want[0] & 0x7F80) == 0)
+    for gg, ww in zip(got, want):
+        assert np.array_equal(gg, ww), [(i, hex(gg[i]), hex(ww[i])) for i in np.flatnonzero(gg != ww)[:8]]
 
 
 @pytest.mark.parametrize("dtype,code,np_t", [("f64", 0, np.float64), ("f32", 1, np.float32),
