@@ -216,6 +216,22 @@ typedef struct kg_poisson_report {
 int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, double tol, int max_outer,
                      int vcycles, kg_poisson_report* report);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * solve as kg_poisson_solve -- the same kernels in the same order, so u and
+ * the report's residuals are bit-identical -- with its multigrid levels and
+ * partial sums in a workspace reserved once per device inside the library
+ * (a static device array of about 17 MB, held until the process ends,
+ * located by kg_init).  It calls no hipMalloc, hipFree or other allocating
+ * routine.  n is limited to KG_POISSON_NOALLOC_MAX_N (larger n: KG_ERR_INVALID).
+ * report->seconds is host wall time from the first residual pass to the end.
+ * Every call on a device shares that device's workspace: calls that overlap
+ * on one device (two host threads, or two contexts on the same device) race
+ * and must be serialised by the caller. */
+#define KG_POISSON_NOALLOC_MAX_N 1024
+int kg_poisson_solve_noalloc(kg_ctx* ctx, int n, const double* f, double* u, double tol,
+                             int max_outer, int vcycles, kg_poisson_report* report);
+
 /* FP64 statistics of a device field x of n values of type dtype, optionally
  * against a reference field ref of type ref_dtype (NULL: none).  out (host,
  * 5 doubles) = { sum x, sum x^2, max |x|, sum (x-ref)^2, max |x-ref| }.

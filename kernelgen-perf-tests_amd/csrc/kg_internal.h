@@ -26,6 +26,13 @@ struct kg_ctx {
 
 // Bytes of the context's kg_field_stats scratch.
 size_t kg_stats_scratch_bytes();
+// The kernels of kg_field_stats on stream st, with their partials in
+// scratch (kg_stats_scratch_bytes() bytes); the 5 results land in the last
+// 5 doubles of scratch.  No timing, no copy, no synchronisation.
+int kg_field_stats_launch(hipStream_t st, int dtype, size_t n, const void* x, int ref_dtype,
+                          const void* ref, double* scratch);
+// Per-device set-up of the Poisson solver's static workspace (kg_init).
+int kg_poisson_static_setup(int device);
 
 // Records the most recent error message (thread-local) and returns the code.
 int kg_set_error(int code, const char* fmt, ...);

@@ -26,8 +26,9 @@
 // no cache line is written in part.
 #include "kg_internal.h"
 
+#include <chrono>
 #include <cmath>
-#include <vector>
+#include <mutex>
 
 namespace {
 
@@ -404,10 +405,10 @@ void smooth(hipStream_t st, Level& L, int n, double omega)
     }
 }
 
-void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post, double omega)
+void vcycle(hipStream_t st, Level* lv, int nl, int l, int pre, int post, double omega)
 {
     Level& L = lv[l];
-    if (l + 1 == lv.size()) {
+    if (l + 1 == nl) {
         // 3 x 3 vertices: (A u)(1,1) = 20 u / (6 h^2) = f
         hipLaunchKernelGGL(pj_exact3_k, dim3(1), dim3(64), 0, st, L.P,
                            (float)(6.0 * L.h * L.h / 20.0), L.f, L.u);
@@ -416,11 +417,11 @@ void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post,
     smooth(st, L, pre, omega);
     Level& C = lv[l + 1];
     // C.f = restricted residual, C.u = 0; C.t needs no clearing: every sweep
-    // rewrites its rows 1 .. N-2 whole and its boundary rows stay 0 from
-    // allocation
+    // rewrites its rows 1 .. N-2 whole and its boundary rows stay 0 from the
+    // workspace's set-up
     hipLaunchKernelGGL(pj_resrestrict_k, grid2(C.N), dim3(PT, PT), 0, st, C.N, C.P, L.P,
                        (float)(1.0 / (6.0 * L.h * L.h)), L.u, L.f, C.f, C.u);
-    vcycle(st, lv, l + 1, pre, post, omega);
+    vcycle(st, lv, nl, l + 1, pre, post, omega);
     {
         const int ntx = (L.P + 255) / 256;
         const unsigned blocks = (unsigned)ntx * (unsigned)((L.N - 2 + 3) / 4);
@@ -429,101 +430,269 @@ void vcycle(hipStream_t st, std::vector<Level>& lv, size_t l, int pre, int post,
     smooth(st, L, post, omega);
 }
 
-}  // namespace
+// Levels n, n/2, ..., 2 cells per side: at most 14 for n <= 16384.
+constexpr int MAX_LEVELS = 14;
+constexpr int MAX_N = 1 << 14;
 
-extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, double tol,
-                                int max_outer, int vcycles, kg_poisson_report* rep)
+// Device memory of one solve, carved in a fixed order from one or more
+// regions: per level (finest first) u, t and f (N x P floats each), then the
+// residual pass's partial sums (res_blocks + 1 doubles), then the partials of
+// the norm of f (kg_stats_scratch_bytes()).  Every block starts at a
+// multiple of 256 bytes.
+struct Workspace {
+    int nl;
+    Level lv[MAX_LEVELS];
+    int res_ntx, res_blocks;
+    double* part;
+    double* stats;
+};
+
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr int pitch4(int N) { return (N + 3) / 4 * 4; }
+constexpr int res_blocks_of(int n)
+{
+    return (pitch4(n + 1) + 255) / 256 * ((n + 1 + RES_RY - 1) / RES_RY);
+}
+
+// Bytes of the workspace of an n-cell solve (n a power of two >= 2).
+constexpr size_t workspace_bytes(int n, size_t stats_bytes)
+{
+    size_t b = 0;
+    for (int m = n; m >= 2; m /= 2) {
+        const size_t ne = (size_t)(m + 1) * (size_t)pitch4(m + 1);
+        b += 3 * align256(ne * sizeof(float));
+    }
+    b += align256((size_t)(res_blocks_of(n) + 1) * sizeof(double));
+    b += align256(stats_bytes);
+    return b;
+}
+
+// Level sizes of an n-cell solve; pointers left null.
+void workspace_shape(Workspace& ws, int n)
+{
+    ws.nl = 0;
+    for (int m = n; m >= 2; m /= 2)
+        ws.lv[ws.nl++] = Level{m + 1, pitch4(m + 1), 1.0 / m, nullptr, nullptr, nullptr};
+    ws.res_ntx = (ws.lv[0].P + 255) / 256;
+    ws.res_blocks = res_blocks_of(n);
+    ws.part = ws.stats = nullptr;
+}
+
+// Points the workspace's blocks into the region at base (workspace_bytes(n)
+// bytes, 256-byte aligned).
+void workspace_carve(Workspace& ws, char* base)
+{
+    size_t off = 0;
+    for (int l = 0; l < ws.nl; l++) {
+        Level& L = ws.lv[l];
+        const size_t b = align256((size_t)L.N * L.P * sizeof(float));
+        for (float** p : {&L.u, &L.t, &L.f}) {
+            *p = (float*)(base + off);
+            off += b;
+        }
+    }
+    ws.part = (double*)(base + off);
+    off += align256((size_t)(ws.res_blocks + 1) * sizeof(double));
+    ws.stats = (double*)(base + off);
+}
+
+// The levels' arrays start at 0: sweeps rely on boundary rows and padding
+// that they never write.
+void workspace_clear(hipStream_t st, Workspace& ws)
+{
+    for (int l = 0; l < ws.nl; l++) {
+        Level& L = ws.lv[l];
+        const long long ne = (long long)L.N * L.P;
+        for (float* p : {L.u, L.t, L.f})
+            hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(ne)), dim3(256), 0, st, ne, p);
+    }
+}
+
+int check_args(kg_ctx* ctx, const char* who, int n, int nmax, const double* f, const double* u)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
-    if (n < 2 || (n & (n - 1)) || n > (1 << 14))
-        return kg_set_error(KG_ERR_INVALID, "kg_poisson_solve: n = %d must be a power of two in [2, 16384]", n);
-    if (!f || !u) return kg_set_error(KG_ERR_INVALID, "kg_poisson_solve: null buffer");
+    if (n < 2 || (n & (n - 1)) || n > nmax)
+        return kg_set_error(KG_ERR_INVALID, "%s: n = %d must be a power of two in [2, %d]", who, n, nmax);
+    if (!f || !u) return kg_set_error(KG_ERR_INVALID, "%s: null buffer", who);
     if ((const void*)f == (const void*)u)
-        return kg_set_error(KG_ERR_INVALID, "kg_poisson_solve: f and u must be distinct");
-    if (max_outer < 1) max_outer = 1;
-    if (vcycles < 1) vcycles = 1;
+        return kg_set_error(KG_ERR_INVALID, "%s: f and u must be distinct", who);
+    return KG_OK;
+}
+
+// The defect-correction loop on a carved and cleared workspace; the stream
+// is synchronised on return.  Timing (events or the host clock) is the
+// caller's: t_start is called once the norm of f is known, right before the
+// first residual pass.
+template <class Start>
+int solve(kg_ctx* ctx, int n, const double* f, double* u, double tol, int max_outer, int vcycles,
+          Workspace& ws, kg_poisson_report* rep, int* outer_out, int* cycles_out, double* rel_out,
+          Start&& t_start)
+{
     hipStream_t st = ctx->stream;
     const int N = n + 1;
     const long long np = (long long)N * N;
     const double h = 1.0 / n;
     const double omega = 0.8;
     const int pre = 2, post = 2;
-    std::vector<Level> lv;
-    for (int m = n; m >= 2; m /= 2)
-        lv.push_back(Level{m + 1, (m + 1 + 3) / 4 * 4, 1.0 / m, nullptr, nullptr, nullptr});
-    std::vector<void*> allocs;
-    auto fail = [&](int rc) {
-        (void)hipStreamSynchronize(st);
-        for (void* p : allocs) (void)hipFree(p);
-        return rc;
-    };
-    auto alloc = [&](size_t bytes, void** p) -> int {
-        hipError_t e = hipMalloc(p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return kg_set_error(KG_ERR_NOMEM, "kg_poisson_solve: %zu bytes: %s", bytes, hipGetErrorString(e));
-        }
-        allocs.push_back(*p);
-        return KG_OK;
-    };
-    for (auto& L : lv) {
-        const long long ne = (long long)L.N * L.P;
-        for (float** p : {&L.u, &L.t, &L.f}) {
-            if (int rc = alloc((size_t)ne * sizeof(float), (void**)p)) return fail(rc);
-            hipLaunchKernelGGL(pj_zero_k, dim3(blocks1(ne)), dim3(256), 0, st, ne, *p);
-        }
-    }
-    // partial sums of r^2 of the fused residual (one per workgroup, then the total)
-    const int res_ntx = (lv[0].P + 255) / 256;
-    const int res_blocks = res_ntx * ((N + RES_RY - 1) / RES_RY);
-    double* part = nullptr;
-    if (int rc = alloc((size_t)(res_blocks + 1) * sizeof(double), (void**)&part)) return fail(rc);
-
-    kg_register_kernel("poisson_smooth", (const void*)pj_smooth4_k);
+    if (max_outer < 1) max_outer = 1;
+    if (vcycles < 1) vcycles = 1;
     double st_f[5];
-    if (int rc = kg_field_stats(ctx, KG_F64, (size_t)np, f, KG_F64, nullptr, st_f)) return fail(rc);
+    if (int rc = kg_field_stats_launch(st, KG_F64, (size_t)np, f, KG_F64, nullptr, ws.stats)) return rc;
+    KG_HIP(hipMemcpyAsync(st_f, ws.stats + kg_stats_scratch_bytes() / sizeof(double) - 5,
+                          sizeof st_f, hipMemcpyDeviceToHost, st));
+    KG_HIP(hipStreamSynchronize(st));
     const double fnorm = std::sqrt(st_f[1]);
     int outer = 0, cycles = 0;
     double rel = 0;
-    hipEvent_t e0, e1;
-    KG_HIP(hipEventCreate(&e0));
-    KG_HIP(hipEventCreate(&e1));
-    (void)hipEventRecord(e0, st);
-    Level& F = lv[0];
+    Level& F = ws.lv[0];
+    t_start();
     for (;;) {
         // r = f - A u in FP64: its FP32 copy is the right-hand side of the
         // inner solve (F.f), its squared norm decides the stop
-        hipLaunchKernelGGL(pj_residual64_k, dim3(res_blocks), dim3(256), 0, st, N, F.P, res_ntx,
-                           1.0 / (6.0 * h * h), (const double*)u, f, F.f, F.u, part);
-        hipLaunchKernelGGL(pj_sum_k, dim3(1), dim3(1024), 0, st, res_blocks, part);
+        hipLaunchKernelGGL(pj_residual64_k, dim3(ws.res_blocks), dim3(256), 0, st, N, F.P, ws.res_ntx,
+                           1.0 / (6.0 * h * h), (const double*)u, f, F.f, F.u, ws.part);
+        hipLaunchKernelGGL(pj_sum_k, dim3(1), dim3(1024), 0, st, ws.res_blocks, ws.part);
         double sumsq = 0.0;
-        KG_HIP(hipMemcpyAsync(&sumsq, part + res_blocks, sizeof(double), hipMemcpyDeviceToHost, st));
+        KG_HIP(hipMemcpyAsync(&sumsq, ws.part + ws.res_blocks, sizeof(double), hipMemcpyDeviceToHost, st));
         KG_HIP(hipStreamSynchronize(st));
         rel = fnorm > 0 ? std::sqrt(sumsq) / fnorm : 0.0;
         if (rep && outer < KG_POISSON_HISTORY) rep->history[outer] = rel;
         if (rel < tol || outer >= max_outer) break;
         // inner FP32 solve of A e = r from e = 0 (F.u, set by the residual pass)
-        for (int c = 0; c < vcycles; c++) vcycle(st, lv, 0, pre, post, omega);
+        for (int c = 0; c < vcycles; c++) vcycle(st, ws.lv, ws.nl, 0, pre, post, omega);
         cycles += vcycles;
         hipLaunchKernelGGL(pj_add_k, dim3(blocks1(np)), dim3(256), 0, st, F.N, F.P, (const float*)F.u, u);
         outer++;
     }
-    (void)hipEventRecord(e1, st);
-    float ms = 0;
-    (void)hipEventSynchronize(e1);
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     KG_HIP(hipGetLastError());
+    *outer_out = outer;
+    *cycles_out = cycles;
+    *rel_out = rel;
+    return KG_OK;
+}
+
+int finish(const char* who, kg_poisson_report* rep, const Workspace& ws, int outer, int cycles,
+           double rel, double seconds, double tol)
+{
     if (rep) {
         rep->outer = outer;
         rep->vcycles = cycles;
         rep->rel_residual = rel;
-        rep->seconds = ms * 1e-3;
-        rep->levels = (int)lv.size();
+        rep->seconds = seconds;
+        rep->levels = ws.nl;
     }
-    return fail(rel < tol ? KG_OK : kg_set_error(KG_ERR_STATE, "kg_poisson_solve: relative residual "
-                                                 "%.3e after %d corrections (tol %.1e)", rel, outer, tol));
+    return rel < tol ? KG_OK : kg_set_error(KG_ERR_STATE, "%s: relative residual %.3e after %d "
+                                            "corrections (tol %.1e)", who, rel, outer, tol);
+}
+
+// ---- the static workspace of kg_poisson_solve_noalloc ----------------------
+// Sized for the largest grid that entry point accepts; one copy per device
+// (a __device__ array exists once on every GPU the code object is loaded
+// on), held until the process ends.
+constexpr int NOALLOC_MAX_N = KG_POISSON_NOALLOC_MAX_N;
+constexpr size_t STATS_BYTES = (1024 * 5 + 5) * sizeof(double);   // kg_stats_scratch_bytes()
+constexpr size_t NOALLOC_BYTES = workspace_bytes(NOALLOC_MAX_N, STATS_BYTES);
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[MAX_DEVICES];   // device address of `scratch` on each device, once set up
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Called by kg_init with `device` current: looks up this device's copy of
+// the static workspace once.  Host code uses only the address found here;
+// `scratch` itself names a host-side stand-in.
+int kg_poisson_static_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES)
+        return kg_set_error(KG_ERR_NODEV, "kg_poisson_static_setup: device %d beyond %d", device, MAX_DEVICES);
+    if (kg_stats_scratch_bytes() != STATS_BYTES)
+        return kg_set_error(KG_ERR_STATE, "kg_poisson_static_setup: statistics scratch size mismatch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return KG_OK;
+    void* p = nullptr;
+    KG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p || ((uintptr_t)p & 255))
+        return kg_set_error(KG_ERR_HIP, "kg_poisson_static_setup: bad workspace address %p", p);
+    g_scratch_ptr[device] = (char*)p;
+    return KG_OK;
+}
+
+extern "C" int kg_poisson_solve(kg_ctx* ctx, int n, const double* f, double* u, double tol,
+                                int max_outer, int vcycles, kg_poisson_report* rep)
+{
+    if (int rc = check_args(ctx, "kg_poisson_solve", n, MAX_N, f, u)) return rc;
+    KG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    Workspace ws;
+    workspace_shape(ws, n);
+    const size_t bytes = workspace_bytes(n, kg_stats_scratch_bytes());
+    char* base = nullptr;
+    hipError_t e = hipMalloc((void**)&base, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return kg_set_error(KG_ERR_NOMEM, "kg_poisson_solve: %zu bytes: %s", bytes, hipGetErrorString(e));
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // every exit from here on waits for the stream and releases what was taken
+    auto done = [&](int rc) {
+        (void)hipStreamSynchronize(st);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(base);
+        return rc;
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        (void)hipGetLastError();
+        return done(kg_set_error(KG_ERR_HIP, "kg_poisson_solve: hipEventCreate failed"));
+    }
+    workspace_carve(ws, base);
+    workspace_clear(st, ws);
+    kg_register_kernel("poisson_smooth", (const void*)pj_smooth4_k);
+    int outer = 0, cycles = 0;
+    double rel = 0;
+    if (int rc = solve(ctx, n, f, u, tol, max_outer, vcycles, ws, rep, &outer, &cycles, &rel,
+                       [&] { (void)hipEventRecord(e0, st); }))
+        return done(rc);
+    float ms = 0;
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        (void)hipGetLastError();
+        return done(kg_set_error(KG_ERR_HIP, "kg_poisson_solve: event timing failed"));
+    }
+    return done(finish("kg_poisson_solve", rep, ws, outer, cycles, rel, ms * 1e-3, tol));
+}
+
+extern "C" int kg_poisson_solve_noalloc(kg_ctx* ctx, int n, const double* f, double* u, double tol,
+                                        int max_outer, int vcycles, kg_poisson_report* rep)
+{
+    if (int rc = check_args(ctx, "kg_poisson_solve_noalloc", n, NOALLOC_MAX_N, f, u)) return rc;
+    if (ctx->device < 0 || ctx->device >= MAX_DEVICES || !g_scratch_ptr[ctx->device])
+        return kg_set_error(KG_ERR_STATE, "kg_poisson_solve_noalloc: workspace of device %d not set up",
+                            ctx->device);
+    KG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    Workspace ws;
+    workspace_shape(ws, n);
+    if (workspace_bytes(n, STATS_BYTES) > NOALLOC_BYTES)
+        return kg_set_error(KG_ERR_INVALID, "kg_poisson_solve_noalloc: n = %d needs %zu bytes of "
+                            "workspace, more than the %zu reserved", n, workspace_bytes(n, STATS_BYTES),
+                            NOALLOC_BYTES);
+    workspace_carve(ws, g_scratch_ptr[ctx->device]);
+    // the previous call's contents are still there
+    workspace_clear(st, ws);
+    kg_register_kernel("poisson_smooth", (const void*)pj_smooth4_k);
+    int outer = 0, cycles = 0;
+    double rel = 0;
+    std::chrono::steady_clock::time_point t0;
+    int rc = solve(ctx, n, f, u, tol, max_outer, vcycles, ws, rep, &outer, &cycles, &rel,
+                   [&] { t0 = std::chrono::steady_clock::now(); });
+    (void)hipStreamSynchronize(st);
+    if (rc) return rc;
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return finish("kg_poisson_solve_noalloc", rep, ws, outer, cycles, rel, sec, tol);
 }
 
 int kg_preload_poisson() { return kg_preload_kernel((const void*)pj_smooth4_k); }

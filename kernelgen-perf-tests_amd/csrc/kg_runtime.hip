@@ -165,6 +165,10 @@ extern "C" int kg_init(kg_ctx** out, int device)
             return rc;
         }
     }
+    if (int rc = kg_poisson_static_setup(device)) {
+        kg_destroy(c);
+        return rc;
+    }
     *out = c;
     return KG_OK;
 }

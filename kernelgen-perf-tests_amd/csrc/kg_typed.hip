@@ -409,20 +409,26 @@ extern "C" int kg_field_stats(kg_ctx* ctx, int dtype, size_t n, const void* x, i
                "kg_field_stats: unknown reference dtype");
     int rc = KG_OK;
     double* dev = ctx->stats;   // KG_STATS_BLOCKS*5 partials followed by 5 results
-    double* res = dev + KG_STATS_BLOCKS * 5;
     kg_register_kernel("field_stats", (const void*)stats_final_k);
     if ((rc = kg_launch_begin(ctx, "field_stats"))) return rc;
-    switch (dtype) {
-    case KG_F64: rc = stats_dispatch_ref<KG_F64>(ctx->stream, ref_dtype, n, x, ref, dev, res); break;
-    case KG_F32: rc = stats_dispatch_ref<KG_F32>(ctx->stream, ref_dtype, n, x, ref, dev, res); break;
-    case KG_BF16: rc = stats_dispatch_ref<KG_BF16>(ctx->stream, ref_dtype, n, x, ref, dev, res); break;
-    default: rc = kg_set_error(KG_ERR_INVALID, "kg_field_stats: unknown dtype %d", dtype);
-    }
-    if (rc) return rc;
+    if ((rc = kg_field_stats_launch(ctx->stream, dtype, n, x, ref_dtype, ref, dev))) return rc;
     if ((rc = kg_launch_end(ctx))) return rc;
-    KG_HIP(hipMemcpyAsync(out, res, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KG_HIP(hipMemcpyAsync(out, dev + KG_STATS_BLOCKS * 5, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
     KG_HIP(hipStreamSynchronize(ctx->stream));
     return KG_OK;
+}
+
+int kg_field_stats_launch(hipStream_t st, int dtype, size_t n, const void* x, int ref_dtype,
+                          const void* ref, double* scratch)
+{
+    double* res = scratch + KG_STATS_BLOCKS * 5;
+    switch (dtype) {
+    case KG_F64: return stats_dispatch_ref<KG_F64>(st, ref_dtype, n, x, ref, scratch, res);
+    case KG_F32: return stats_dispatch_ref<KG_F32>(st, ref_dtype, n, x, ref, scratch, res);
+    case KG_BF16: return stats_dispatch_ref<KG_BF16>(st, ref_dtype, n, x, ref, scratch, res);
+    default: return kg_set_error(KG_ERR_INVALID, "kg_field_stats: unknown dtype %d", dtype);
+    }
 }
 
 size_t kg_stats_scratch_bytes() { return (KG_STATS_BLOCKS * 5 + 5) * sizeof(double); }

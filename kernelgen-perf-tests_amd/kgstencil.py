@@ -270,7 +270,7 @@ _BATCH_SIGS = {
     "kg_batch_fill_grid": [_dp, _i, C.c_ulonglong],
     "kg_batch_step_sequential": [_dp, _i],
 }
-EXPORTS += list(_BATCH_SIGS) + ["kg_serve_run", "kg_poisson_solve"]
+EXPORTS += list(_BATCH_SIGS) + ["kg_serve_run", "kg_poisson_solve", "kg_poisson_solve_noalloc"]
 
 
 def _declare_batch(lib):
@@ -405,15 +405,21 @@ class PoissonReport(C.Structure):
                 ("seconds", _d), ("history", _d * 32)]
 
 
+# largest n of kg_poisson_solve_noalloc (KG_POISSON_NOALLOC_MAX_N)
+POISSON_NOALLOC_MAX_N = 1024
+
+
 def poisson_solve(ctx: Context, n: int, f: DeviceArray, u: DeviceArray, tol: float = 1e-10,
-                  max_outer: int = 20, vcycles: int = 2) -> dict:
-    """kg_poisson_solve: FP32 multigrid inside FP64 defect correction."""
-    fn = ctx.lib.kg_poisson_solve
+                  max_outer: int = 20, vcycles: int = 2, noalloc: bool = False) -> dict:
+    """kg_poisson_solve: FP32 multigrid inside FP64 defect correction.
+    noalloc=True calls kg_poisson_solve_noalloc (same solve, workspace
+    reserved inside the library, n <= POISSON_NOALLOC_MAX_N)."""
+    name = "kg_poisson_solve_noalloc" if noalloc else "kg_poisson_solve"
+    fn = getattr(ctx.lib, name)
     fn.argtypes = [_dp, _i, _dp, _dp, _d, _i, _i, C.POINTER(PoissonReport)]
     fn.restype = _i
     rep = PoissonReport()
-    _check(ctx.lib, fn(ctx.ptr, n, f.ptr, u.ptr, tol, max_outer, vcycles, C.byref(rep)),
-           "kg_poisson_solve")
+    _check(ctx.lib, fn(ctx.ptr, n, f.ptr, u.ptr, tol, max_outer, vcycles, C.byref(rep)), name)
     return {"outer": rep.outer, "vcycles": rep.vcycles, "levels": rep.levels,
             "rel_residual": rep.rel_residual, "seconds": rep.seconds,
             "history": list(rep.history[: min(rep.outer + 1, 32)])}
