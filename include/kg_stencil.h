@@ -149,15 +149,23 @@ int kg_gameoflife(kg_ctx* ctx, int nx, int ny, const double* u0, double* u1);
  * cells this is the FP64 kernel's rule exactly.  Buffers 16-byte aligned. */
 int kg_life_u8(kg_ctx* ctx, int nx, int ny, const uint8_t* c0, uint8_t* c1);
 
-/* ---- the suite's single-precision tests (real = float in the reference) ----- */
-/* vecadd: w2 = w0 + w1 over all nx*ny*ns points (the host rotates w0 <- w1 <-
- * w2 <- w0 between steps).  Buffers 16-byte aligned. */
-int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const float* w0, const float* w1, float* w2);
+/* ---- the suite's non-stencil tests (real = double in the reference:
+ * makefile.in:12 overrides the `real ?= float` of each gcc makefile) ---- */
+/* vecadd/vecadd.c:38-84: w2 = w0 + w1 over all nx*ny*ns points (the host
+ * rotates w0 <- w1 <- w2 <- w0 between steps). */
+int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const double* w0, const double* w1, double* w2);
 /* matvec/matvec.c:38-70: y[j] = sum_i A[i + nx*j] * x[i], i ascending, for
- * j < ny (the 2D test runs at nx x ny*ns). */
-int kg_matvec(kg_ctx* ctx, int nx, int ny, const float* A, const float* x, float* y);
-/* sincos/sincos.F90:14-74: xy = sin(x) + cos(y) over all points. */
-int kg_sincos(kg_ctx* ctx, int nx, int ny, int ns, const float* x, const float* y, float* xy);
+ * j < ny (the 2D test runs at nx x ny*ns); sequential sums, bit-identical to
+ * an IEEE build of the reference. */
+int kg_matvec(kg_ctx* ctx, int nx, int ny, const double* A, const double* x, double* y);
+/* sincos/sincos.F90:14-74 (-fdefault-real-8): xy = sin(x) + cos(y) over all
+ * points, with the device's FP64 sin and cos. */
+int kg_sincos(kg_ctx* ctx, int nx, int ny, int ns, const double* x, const double* y, double* xy);
+/* Single-precision forms of the three, beside them (no reference
+ * counterpart: the reference's `real` is double for these tests). */
+int kg_vecadd_f32(kg_ctx* ctx, int nx, int ny, int ns, const float* w0, const float* w1, float* w2);
+int kg_matvec_f32(kg_ctx* ctx, int nx, int ny, const float* A, const float* x, float* y);
+int kg_sincos_f32(kg_ctx* ctx, int nx, int ny, int ns, const float* x, const float* y, float* xy);
 
 /* ---- precision variants (beside the FP64 drop-in kernels) -------------------
  * No reference counterpart: the reference computes in FP64 only.  Same

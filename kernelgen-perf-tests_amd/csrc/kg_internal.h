@@ -71,7 +71,7 @@ int kg_preload_life();
 int kg_preload_typed();
 int kg_preload_batch();
 int kg_preload_poisson();
-int kg_preload_suite32();
+int kg_preload_vecmath();
 int kg_preload_kernel(const void* func);
 
 // Validation helpers shared by the entry points.

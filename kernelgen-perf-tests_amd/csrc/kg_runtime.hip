@@ -158,7 +158,7 @@ extern "C" int kg_init(kg_ctx** out, int device)
     int (*preload[])() = {kg_preload_stencil3d, kg_preload_tricubic, kg_preload_stencil2d,
                           kg_preload_matmul, kg_preload_util, kg_preload_life,
                           kg_preload_typed, kg_preload_batch, kg_preload_poisson,
-                          kg_preload_suite32};
+                          kg_preload_vecmath};
     for (auto f : preload) {
         if (int rc = f()) {
             kg_destroy(c);

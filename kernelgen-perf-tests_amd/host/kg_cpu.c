@@ -369,18 +369,23 @@ int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, const double
     return KG_OK;
 }
 
-/* ---- the suite's single-precision tests (real = float) ---- */
-int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const float* w0, const float* w1, float* w2)
+/* ---- the suite's non-stencil tests (real = double: makefile.in:12) ---- */
+/* vecadd/vecadd.c:69-84 */
+int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const double* w0, const double* w1, double* w2)
 {
     const double t0_ = now();
-    const size_t n = (size_t)nx * ny * ns;
-    for (size_t i = 0; i < n; i++) w2[i] = w0[i] + w1[i];
+    for (int k = 0; k < ns; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t c = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
+                w2[c] = w0[c] + w1[c];
+            }
     toc(ctx, "vecadd", t0_);
     return KG_OK;
 }
 
 /* matvec/matvec.c:38-70 */
-int kg_matvec(kg_ctx* ctx, int nx, int ny, const float* A, const float* x, float* y)
+int kg_matvec(kg_ctx* ctx, int nx, int ny, const double* A, const double* x, double* y)
 {
     const double t0_ = now();
     for (int j = 0; j < ny; j++) {
@@ -392,11 +397,15 @@ int kg_matvec(kg_ctx* ctx, int nx, int ny, const float* A, const float* x, float
 }
 
 /* sincos/sincos.F90:62-70 */
-int kg_sincos(kg_ctx* ctx, int nx, int ny, int ns, const float* x, const float* y, float* xy)
+int kg_sincos(kg_ctx* ctx, int nx, int ny, int ns, const double* x, const double* y, double* xy)
 {
     const double t0_ = now();
-    const size_t n = (size_t)nx * ny * ns;
-    for (size_t i = 0; i < n; i++) xy[i] = sinf(x[i]) + cosf(y[i]);
+    for (int k = 0; k < ns; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t c = (size_t)i + (size_t)nx * j + (size_t)nx * ny * k;
+                xy[c] = sin(x[c]) + cos(y[c]);
+            }
     toc(ctx, "sincos", t0_);
     return KG_OK;
 }

@@ -42,6 +42,9 @@ SIGNATURES = {
     "kg_vecadd": [_i, _i, _i, _dp, _dp, _dp],
     "kg_matvec": [_i, _i, _dp, _dp, _dp],
     "kg_sincos": [_i, _i, _i, _dp, _dp, _dp],
+    "kg_vecadd_f32": [_i, _i, _i, _dp, _dp, _dp],
+    "kg_matvec_f32": [_i, _i, _dp, _dp, _dp],
+    "kg_sincos_f32": [_i, _i, _i, _dp, _dp, _dp],
     "kg_laplacian_f32": [_i, _i, _i, _f, _f, _dp, _dp],
     "kg_laplacian_bf16": [_i, _i, _i, _f, _f, _dp, _dp],
     "kg_wave13pt_f32": [_i, _i, _i, _f, _f, _f, _dp, _dp, _dp],

@@ -18,7 +18,10 @@
 #
 # Each C test file is compiled with -Dmain=kgref_main_<test> so that its kernel
 # function can be called directly; nothing is copied or stubbed.  The Fortran
-# kernels (jacobi, matmul) are built with amdflang, gfortran being absent.
+# kernels (jacobi, matmul, sincos) are built with amdflang, gfortran being
+# absent.  Every test is built as the reference builds it: real = double
+# (makefile.in:12, which overrides the `real ?= float` of the vecadd, matvec
+# and sincos makefiles) and -fdefault-real-8 for Fortran.
 #
 # -march defaults to x86-64-v4 (AVX-512) rather than native so that the
 # prebuilt files run on any current x86-64 host; set REF_MARCH=native to match
@@ -35,8 +38,8 @@ if [ ! -d "$REF" ]; then
     exit 0
 fi
 
-CTESTS="laplacian divergence gradient wave13pt uxx1 tricubic tricubic2 lapgsrb gaussblur whispering gameoflife"
-FTESTS="jacobi matmul"
+CTESTS="laplacian divergence gradient wave13pt uxx1 tricubic tricubic2 lapgsrb gaussblur whispering gameoflife vecadd matvec"
+FTESTS="jacobi matmul sincos"
 
 rm -rf "$OUT"
 mkdir -p "$OUT/ieee" "$OUT/fast" "$OUT/bin"
@@ -63,23 +66,10 @@ for t in $FTESTS; do
     amdflang -cpp -fdefault-real-8 -fPIC $FAST -c "$REF/$t/$t.F90" -o "$OUT/fast/${t}_f.o"
     gcc $COMMON $IEEE -I"$REF/$t" -Dmain=kgref_main_$t -c "$REF/$t/main.c" -o "$OUT/ieee/${t}_main.o"
     gcc $COMMON $FAST -I"$REF/$t" -c "$REF/$t/main.c" -o "$OUT/fast/${t}_prog.o"
-    gcc "$OUT/fast/${t}_prog.o" "$OUT/fast/${t}_f.o" "$OUT/fast/timing.o" -o "$OUT/bin/$t" -lrt
+    gcc "$OUT/fast/${t}_prog.o" "$OUT/fast/${t}_f.o" "$OUT/fast/timing.o" -o "$OUT/bin/$t" -lrt -lm
 done
 
-# Single-precision tests (real = float, as their gcc makefiles default):
-# matvec (C) and sincos (C main + Fortran kernel, amdflang).
-C32="-std=c99 -D_GNU_SOURCE -Dreal=float -fPIC -w"
-gcc $C32 $IEEE -I"$REF/matvec" -Dmain=kgref_main_matvec -c "$REF/matvec/matvec.c" -o "$OUT/ieee/matvec.o"
-gcc $C32 $FAST -I"$REF/matvec" "$REF/matvec/matvec.c" "$REF/matvec/timing.c" -o "$OUT/bin/matvec" -lrt
-amdflang -cpp -fPIC $IEEE -c "$REF/sincos/sincos.F90" -o "$OUT/ieee/sincos_f.o"
-amdflang -cpp -fPIC $FAST -c "$REF/sincos/sincos.F90" -o "$OUT/fast/sincos_f32_f.o"
-gcc $C32 $FAST -I"$REF/sincos" -c "$REF/sincos/main.c" -o "$OUT/fast/sincos_prog.o"
-gcc $C32 $FAST -c "$REF/sincos/timing.c" -o "$OUT/fast/sincos_timing_prog.o"
-gcc "$OUT/fast/sincos_prog.o" "$OUT/fast/sincos_f32_f.o" "$OUT/fast/sincos_timing_prog.o" \
-    -o "$OUT/bin/sincos" -lrt -lm
-rm -f "$OUT/fast/sincos_f32_f.o" "$OUT/fast/sincos_timing_prog.o"
-
 gcc -shared -o "$OUT/libref_ieee.so" "$OUT"/ieee/*.o -lrt -lm
-gcc -shared -o "$OUT/libref_fast.so" $(ls "$OUT"/fast/*.o | grep -v _prog.o) -lrt
+gcc -shared -o "$OUT/libref_fast.so" $(ls "$OUT"/fast/*.o | grep -v _prog.o) -lrt -lm
 rm -rf "$OUT/ieee" "$OUT/fast"
 echo "build_ref.sh: built $OUT (march=$MARCH)"

@@ -465,17 +465,36 @@ DEF_WAVE(bf16, uint16_t, LD_BF16, ST_BF16)
 DEF_GAUSS3(f64, double)
 DEF_GAUSS3(f32, float)
 
-/* ---- the suite's single-precision tests (real = float) ------------------ */
+/* ---- the suite's non-stencil tests (real = double: makefile.in:12) ------ */
 
-/* vecadd: w2 = w0 + w1 (the reference's vecadd source is not in its tree:
- * vecadd/<target>/makefile builds ../vecadd.c, which is absent; this is the
- * operation the test's name states). */
+/* vecadd/vecadd.c:69-84: w2 = w0 + w1 over every point. */
+void oracle_vecadd(size_t n, const double* w0, const double* w1, double* w2)
+{
+    for (size_t i = 0; i < n; i++) w2[i] = w0[i] + w1[i];
+}
+
+/* matvec/matvec.c:59-69: y[j] = 0; y[j] += A(i,j) * x[i], i ascending. */
+void oracle_matvec(int nx, int ny, const double* A, const double* x, double* y)
+{
+    for (int j = 0; j < ny; j++) {
+        y[j] = 0;
+        for (int i = 0; i < nx; i++) y[j] += A[IDX2(i, j)] * x[i];
+    }
+}
+
+/* sincos/sincos.F90:62-70 with -fdefault-real-8: xy = sin(x) + cos(y). */
+void oracle_sincos(size_t n, const double* x, const double* y, double* xy)
+{
+    for (size_t i = 0; i < n; i++) xy[i] = sin(x[i]) + cos(y[i]);
+}
+
+/* Single-precision forms of the same three (the library's _f32 entry
+ * points; the reference does not build these). */
 void oracle_vecadd_f32(size_t n, const float* w0, const float* w1, float* w2)
 {
     for (size_t i = 0; i < n; i++) w2[i] = w0[i] + w1[i];
 }
 
-/* matvec/matvec.c:60-69: y[j] = 0; y[j] += A(i,j) * x[i], i ascending. */
 void oracle_matvec_f32(int nx, int ny, const float* A, const float* x, float* y)
 {
     for (int j = 0; j < ny; j++) {
@@ -484,7 +503,6 @@ void oracle_matvec_f32(int nx, int ny, const float* A, const float* x, float* y)
     }
 }
 
-/* sincos/sincos.F90:62-70: xy = sin(x) + cos(y) in single precision. */
 void oracle_sincos_f32(size_t n, const float* x, const float* y, float* xy)
 {
     for (size_t i = 0; i < n; i++) xy[i] = sinf(x[i]) + cosf(y[i]);

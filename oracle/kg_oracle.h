@@ -111,8 +111,12 @@ void oracle_gaussblur3_f64(int nx, int ny, int ns, double s0, double s1, double 
 void oracle_gaussblur3_f32(int nx, int ny, int ns, float s0, float s1, float s2,
                            float s4, float s5, float s8, const float* w0, float* w1);
 
-/* The suite's single-precision tests: vecadd (w2 = w0 + w1), matvec
- * (y = A x, sequential row sums), sincos (xy = sinf(x) + cosf(y)). */
+/* The suite's non-stencil tests in FP64 (the reference's real = double):
+ * vecadd (w2 = w0 + w1), matvec (y = A x, sequential row sums), sincos
+ * (xy = sin(x) + cos(y)); and their single-precision forms. */
+void oracle_vecadd(size_t n, const double* w0, const double* w1, double* w2);
+void oracle_matvec(int nx, int ny, const double* A, const double* x, double* y);
+void oracle_sincos(size_t n, const double* x, const double* y, double* xy);
 void oracle_vecadd_f32(size_t n, const float* w0, const float* w1, float* w2);
 void oracle_matvec_f32(int nx, int ny, const float* A, const float* x, float* y);
 void oracle_sincos_f32(size_t n, const float* x, const float* y, float* xy);

@@ -38,6 +38,50 @@ def written_roles(spec: K.Spec):
     return spec.pair
 
 
+VEC_CASES = [(19, 7, 6), (66, 5, 5)]
+MATVEC_CASES = [(37, 13), (70, 7)]
+
+
+def vecmath_fixture(ref) -> dict:
+    """vecadd (3-level rotation, nt = 1 and 3), matvec and sincos (same
+    inputs every step: one call) from the reference's FP64 build."""
+    import ctypes as C
+    lib = ref.lib
+    vp = lambda a: C.c_void_p(a.ctypes.data)
+    payload, meta = {}, {"vecadd": [], "matvec": [], "sincos": []}
+    for ci, dims in enumerate(VEC_CASES):
+        rng = np.random.default_rng(2000 + ci)
+        n = int(np.prod(dims))
+        w = [rng.uniform(-1, 1, n) for _ in range(3)]
+        for r, a in zip(("w0", "w1", "w2"), w):
+            payload[f"vecadd_c{ci}_in_{r}"] = a
+        for nt in STEPS:
+            st = [a.copy() for a in w]
+            for _ in range(nt):
+                lib.vecadd(C.c_int(dims[0]), C.c_int(dims[1]), C.c_int(dims[2]),
+                           vp(st[0]), vp(st[1]), vp(st[2]))
+                st = [st[1], st[2], st[0]]
+            for r, a in zip(("w0", "w1", "w2"), st):
+                payload[f"vecadd_c{ci}_nt{nt}_{r}"] = a
+        meta["vecadd"].append({"dims": list(dims)})
+        x, y = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        xy = rng.uniform(-1, 1, n)
+        payload[f"sincos_c{ci}_in_x"], payload[f"sincos_c{ci}_in_y"] = x, y
+        d = [C.c_int(v) for v in dims]
+        lib.sincos_(C.byref(d[0]), C.byref(d[1]), C.byref(d[2]), vp(x), vp(y), vp(xy))
+        payload[f"sincos_c{ci}_xy"] = xy
+        meta["sincos"].append({"dims": list(dims)})
+    for ci, (nx, ny) in enumerate(MATVEC_CASES):
+        rng = np.random.default_rng(3000 + ci)
+        A, x, y = rng.uniform(-1, 1, nx * ny), rng.uniform(-1, 1, nx), rng.uniform(-1, 1, ny)
+        payload[f"matvec_c{ci}_in_A"], payload[f"matvec_c{ci}_in_x"] = A, x
+        lib.matvec(C.c_int(nx), C.c_int(ny), vp(A), vp(x), vp(y))
+        payload[f"matvec_c{ci}_y"] = y
+        meta["matvec"].append({"dims": [nx, ny]})
+    payload["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    return payload
+
+
 def main() -> int:
     if not K.REF_IEEE_LIB.exists():
         print(f"{K.REF_IEEE_LIB} missing: run `make -C oracle ref` first", file=sys.stderr)
@@ -61,9 +105,12 @@ def main() -> int:
         payload["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
         np.savez_compressed(HERE / f"{name}.npz", **payload)
         index[name] = {"cases": meta, "steps": list(STEPS), "written": list(written_roles(spec))}
+    vm = vecmath_fixture(ref)
+    np.savez_compressed(HERE / "vecmath.npz", **vm)
+    index["vecmath"] = json.loads(bytes(vm["meta"]).decode())
     (HERE / "index.json").write_text(json.dumps(
         {"source": "oracle/_ref/libref_ieee.so (reference sources, -O2 -fno-fast-math "
-                   "-ffp-contract=off; jacobi/matmul via amdflang)",
+                   "-ffp-contract=off; jacobi/matmul/sincos via amdflang; real = double)",
          "kernels": index}, indent=1))
     print(f"wrote {len(index)} fixtures to {HERE}")
     return 0

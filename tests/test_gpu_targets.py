@@ -30,7 +30,7 @@ def means(out):
     return {k: float(v) for k, v in re.findall(r"^(initial mean|final mean) = (\S+)", out, re.M)}
 
 
-@pytest.mark.parametrize("name", ALL + ["vecadd"])
+@pytest.mark.parametrize("name", ALL)
 def test_hip_program_matches_gcc_program(built, name):
     print(f"hip target {name} {args_for(name)}", flush=True)
     h = run(K.ROOT / name / "hip" / name, args_for(name), {"NO_TIMING": "1"})

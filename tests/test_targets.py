@@ -14,12 +14,9 @@ import pytest
 import kgspec as K
 
 TESTS3 = ["laplacian", "divergence", "gradient", "wave13pt", "uxx1", "tricubic", "tricubic2",
-          "lapgsrb", "matmul", "sincos"]
+          "lapgsrb", "matmul", "sincos", "vecadd"]
 TESTS2 = ["jacobi", "gaussblur", "whispering", "gameoflife", "matvec"]
 ALL = TESTS3 + TESTS2
-# vecadd has no program in the reference tree (its makefile names a missing
-# ../vecadd.c); it is checked against the oracle and its own contract only.
-OWN_ONLY = ["vecadd"]
 
 
 def args_for(name, small=True):
@@ -82,7 +79,7 @@ def test_benchmark_driver_table(gcc_built):
     out = subprocess.run([str(K.ROOT / "benchmark"), "34", "18", "12", "2", "1", "gcc"],
                          capture_output=True, text=True, cwd=K.ROOT, timeout=600).stdout
     rows = {m.group(1): m.group(0) for m in re.finditer(r"^\|\s+(\w+) \|\s+gcc \|.*$", out, re.M)}
-    assert sorted(rows) == sorted(ALL + OWN_ONLY)
+    assert sorted(rows) == sorted(ALL)
     for name, row in rows.items():
         cells = [c.strip() for c in row.strip("|").split("|")]
         assert "FAIL" not in cells, row
