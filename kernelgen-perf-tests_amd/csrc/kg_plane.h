@@ -26,9 +26,13 @@
 // elements apart and planes `sz` apart (sy = nx, sz = nx*ny for the dense
 // layout of the C ABI; batched grids pad sy to a multiple of VEC).
 //
-// Measured on MI355X at 512x256x256 FP64 (cold caches): laplacian 102 us with
-// this engine against 143 us for per-thread z-marching with cache-served
-// neighbours; HBM reads 1.06x the input size (the y-halo rows are L2 hits).
+// Measured on MI355X at 512x256x256 FP64 (cold caches): laplacian 99-102 us
+// with this engine against 143 us for per-thread z-marching with
+// cache-served neighbours.  HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE
+// against the algorithmic bytes) is 1.03x for wave13pt but 1.18x for
+// laplacian and 1.22x for lapgsrb: tiles 512 bytes wide re-fetch the x-halo
+// lines their neighbours on other XCDs also read, and the y-halo rows of a
+// tile are read again by the tiles above and below it.
 #pragma once
 
 #include "kg_internal.h"
@@ -145,7 +149,7 @@ __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned off,
 
 // ---------------------------------------------------------------- config
 template <int VEC_, int W_, int H_, int LL_, int LR_, int ZL_, int ZR_, int D_, int NAUX_, int KC_,
-          class T_ = F64, int LPR_ = 64, bool FULL_ = false>
+          class T_ = F64, int LPR_ = 64, bool FULL_ = false, bool XS_ = false>
 struct Cfg {
     using T = T_;
     using S = typename T::S;           // storage type
@@ -165,6 +169,13 @@ struct Cfg {
     // the centre value of the input level), so no cache line of the row is
     // written in part; for fields whose levels share their boundary values.
     static constexpr bool FULL = FULL_;
+    // XS: x-neighbours along the lane's own row (any plane of the register
+    // ring) come from the neighbouring lanes' registers through wave-wide DPP
+    // shifts (View::xat) instead of LDS.  The first lane of the row then holds
+    // the H halo values left of the tile and the last lane the H values right
+    // of it.  One row per wave (LPR = 64).
+    static constexpr bool XS = XS_;
+    static constexpr int NH = XS ? H : 1;             // halo values per ring slot
     static constexpr int RPW = 64 / LPR;             // rows per wave
     static constexpr int ROWS = W * RPW;             // tile rows, halo rows included
     static constexpr int TXW = LPR * VEC;            // tile width in points
@@ -181,13 +192,42 @@ struct Cfg {
     static_assert(LPR == 64 || LPR == 32 || LPR == 16, "lanes per row");
     static_assert(LPR > 2 * H, "halo lanes of a row must not overlap");
     static_assert(TY > 0, "tile has no computed rows");
+    static_assert(!XS || (LPR == 64 && H <= 2 * VEC), "x-shift mode: one row per wave, H <= 2 VEC");
 };
 
-template <class F, int VEC>
+template <class F, int VEC, int NH = 1>
 struct Slot {
     F v[VEC];
-    F h;   // x-halo value carried by the first and last H lanes of a row
+    // x-halo: one value carried by each of the first and last H lanes of a
+    // row, or (x-shift mode) H values on the first and on the last lane
+    F h[NH];
 };
+
+// Wave-wide DPP lane shifts (GFX9 family): wave_shr:1 gives lane i the value
+// of lane i-1, wave_shl:1 that of lane i+1; the lane without a source (0 or
+// 63) keeps `old`.
+constexpr int KG_DPP_WAVE_SHL1 = 0x130;
+constexpr int KG_DPP_WAVE_SHR1 = 0x138;
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double old, double src)
+{
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+    const unsigned long long s = __builtin_bit_cast(unsigned long long, src);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)s, CTRL,
+                                                              0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32),
+                                                              (int)(unsigned)(s >> 32), CTRL, 0xF, 0xF,
+                                                              false);
+    return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float old, float src)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, src), CTRL,
+                                                                 0xF, 0xF, false));
+}
 
 template <class F, int VEC, int NA>
 struct ASlot {
@@ -236,7 +276,7 @@ template <class C, int P = 0>
 struct View {
     using F = typename C::F;
     const F* lds;                              // [NLS][ROWS][VEC][LQ]
-    const Slot<F, C::VEC>* ring;               // [NP]
+    const Slot<F, C::VEC, C::NH>* ring;        // [NP]
     const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
     int k, row, cl;   // plane, tile row and lane-in-row of this lane
     bool wr;          // the lane's points lie in the (padded) row
@@ -256,6 +296,25 @@ struct View {
     __device__ __forceinline__ F col(int v, int dz) const
     {
         return ring[(P + C::ZLO + dz) % C::NP].v[v];
+    }
+    // x-shift mode: main field at (x + v + dx, j, k + dz) from registers,
+    // |dx| <= H, -ZL <= dz <= ZHI.  Point x0 + e of the row (x0 the tile's
+    // first column) is component e mod VEC of lane floor(e / VEC); a value m
+    // lanes away takes m shifts, the row's end lane receiving the halo point
+    // the shift brings in (left halo h[i] = x0 - H + i on lane 0, right halo
+    // h[i] = x0 + TXW + i on lane 63).  All 64 lanes must be active.
+    __device__ __forceinline__ F xat(int v, int dx, int dz) const
+    {
+        static_assert(C::XS, "xat needs the x-shift mode");
+        const Slot<F, C::VEC, C::NH>& s = ring[(P + C::ZLO + dz) % C::NP];
+        const int e = v + dx;
+        const int q = floor_div(e, C::VEC), r = e - q * C::VEC;
+        F t = s.v[r];
+#pragma unroll
+        for (int m = 1; m <= -q; m++) t = dpp_shift<KG_DPP_WAVE_SHR1>(s.h[C::H - m * C::VEC + r], t);
+#pragma unroll
+        for (int m = 1; m <= q; m++) t = dpp_shift<KG_DPP_WAVE_SHL1>(s.h[(m - 1) * C::VEC + r], t);
+        return t;
     }
     __device__ __forceinline__ F side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
 };
@@ -278,13 +337,14 @@ __device__ __forceinline__ void march(int k0, int k1, Load&& load, LoadAux&& loa
                                       Stage&& stage, Compute&& compute)
 {
     using F = typename C::F;
-    Slot<F, C::VEC> ring[C::NP];
+    Slot<F, C::VEC, C::NH> ring[C::NP];
     ASlot<F, C::VEC, C::NA> aux[C::NP];
 #pragma unroll
     for (int r = 0; r < C::NP; r++) {
 #pragma unroll
         for (int v = 0; v < C::VEC; v++) ring[r].v[v] = F(0);
-        ring[r].h = F(0);
+#pragma unroll
+        for (int i = 0; i < C::NH; i++) ring[r].h[i] = F(0);
     }
 #pragma unroll
     for (int r = 0; r < C::RS; r++) load(k0 - C::ZLO + r, ring[r], true);
@@ -296,8 +356,9 @@ __device__ __forceinline__ void march(int k0, int k1, Load&& load, LoadAux&& loa
         if (dz >= -C::ZLO) {
             stage(k0 + dz, ring[C::ZLO + dz]);
         } else {
-            Slot<F, C::VEC> t;
-            t.h = F(0);
+            Slot<F, C::VEC, C::NH> t;
+#pragma unroll
+            for (int i = 0; i < C::NH; i++) t.h[i] = F(0);
 #pragma unroll
             for (int v = 0; v < C::VEC; v++) t.v[v] = F(0);
             load(k0 + dz, t, true);
@@ -349,11 +410,15 @@ __device__ __forceinline__ void plane_tile(typename C::F* lds, int nx, int ny,
     const int x = x0 + c * C::VEC;
     const bool row_ok = j >= 0 && j < ny;
     const bool comp_row = row >= C::H && row < C::ROWS - C::H && j >= LO && j < ny - HI;
-    // x-halo column of this lane (if any)
+    // x-halo columns of this lane (if any): hx + i for i < NH
     int hx = -1;
-    if (c < C::H) hx = x0 - C::H + c;
-    else if (c >= C::LPR - C::H) hx = x0 + C::TXW + (c - (C::LPR - C::H));
-    const bool h_ok = row_ok && hx >= 0 && hx < nx;
+    if constexpr (C::XS) {
+        if (c == 0) hx = x0 - C::H;
+        else if (c == C::LPR - 1) hx = x0 + C::TXW;
+    } else {
+        if (c < C::H) hx = x0 - C::H + c;
+        else if (c >= C::LPR - C::H) hx = x0 + C::TXW + (c - (C::LPR - C::H));
+    }
     bool p_ok[C::VEC];
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) p_ok[v] = row_ok && x + v < nx;
@@ -362,7 +427,12 @@ __device__ __forceinline__ void plane_tile(typename C::F* lds, int nx, int ny,
     // base + k*sz, sz*sizeof(S) bytes (at most 2^31; the launchers check).
     const unsigned plane_bytes = (unsigned)(sz * (long long)sizeof(S));
     const unsigned off_v = p_ok[0] ? (unsigned)(((long long)j * sy + x) * (long long)sizeof(S)) : KG_OOB;
-    const unsigned off_h = h_ok ? (unsigned)(((long long)j * sy + hx) * (long long)sizeof(S)) : KG_OOB;
+    unsigned off_h[C::NH];
+#pragma unroll
+    for (int i = 0; i < C::NH; i++) {
+        const bool h_ok = row_ok && hx >= 0 && hx + i >= 0 && hx + i < nx;
+        off_h[i] = h_ok ? (unsigned)(((long long)j * sy + hx + i) * (long long)sizeof(S)) : KG_OOB;
+    }
     const unsigned off_a = comp_row ? off_v : KG_OOB;
     // Resource for plane k of array base; empty unless want and 0 <= k < ns.
     auto plane_rsrc = [&](const S* base, int k, bool want) {
@@ -375,27 +445,41 @@ __device__ __forceinline__ void plane_tile(typename C::F* lds, int nx, int ny,
     // VEC > 1 every lane with x < nx reads VEC values: callers guarantee that
     // rows hold a multiple of VEC elements (nx % VEC == 0 in the dense
     // layout, padded rows otherwise).
-    auto load = [&](int k, Slot<F, C::VEC>& s, bool want) {
+    auto load = [&](int k, Slot<F, C::VEC, C::NH>& s, bool want) {
         const __amdgpu_buffer_rsrc_t r = plane_rsrc(op.src, k, want);
         bload<T, C::VEC>(r, off_v, s.v);
-        F h[1];
-        bload<T, 1>(r, off_h, h);
-        s.h = h[0];
+#pragma unroll
+        for (int i = 0; i < C::NH; i++) {
+            F h[1];
+            bload<T, 1>(r, off_h[i], h);
+            s.h[i] = h[0];
+        }
     };
     auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s, bool want) {
 #pragma unroll
         for (int a = 0; a < C::NAUX; a++) bload<T, C::VEC>(plane_rsrc(op.side[a], k, want), off_a, s.a[a]);
     };
-    auto stage = [&](int k, const Slot<F, C::VEC>& s) {
+    auto stage = [&](int k, const Slot<F, C::VEC, C::NH>& s) {
         F* L = lds + ((k % C::NLS) * C::ROWS + row) * C::VEC * C::LQ;
 #pragma unroll
         for (int v = 0; v < C::VEC; v++) L[v * C::LQ + C::HL + c] = s.v[v];
-        // the x-halo value of the first and last H lanes: tile column e < 0 or >= TXW
-        const bool lo_h = c < C::H, hi_h = c >= C::LPR - C::H;
-        if (lo_h || hi_h) {
-            const int e = lo_h ? c - C::H : C::TXW + (c - (C::LPR - C::H));
-            const int lq = floor_div(e, C::VEC);
-            L[(e - lq * C::VEC) * C::LQ + C::HL + lq] = s.h;
+        // the x-halo values: tile columns e < 0 or >= TXW
+        if constexpr (C::XS) {
+            if (c == 0 || c == C::LPR - 1) {
+#pragma unroll
+                for (int i = 0; i < C::NH; i++) {
+                    const int e = c == 0 ? i - C::H : C::TXW + i;
+                    const int lq = floor_div(e, C::VEC);
+                    L[(e - lq * C::VEC) * C::LQ + C::HL + lq] = s.h[i];
+                }
+            }
+        } else {
+            const bool lo_h = c < C::H, hi_h = c >= C::LPR - C::H;
+            if (lo_h || hi_h) {
+                const int e = lo_h ? c - C::H : C::TXW + (c - (C::LPR - C::H));
+                const int lq = floor_div(e, C::VEC);
+                L[(e - lq * C::VEC) * C::LQ + C::HL + lq] = s.h[0];
+            }
         }
     };
 
@@ -443,7 +527,7 @@ template <class C, int P = 0>
 struct PView {
     using F = typename C::F;
     const F* lds;                              // guard lanes skipped
-    const Slot<F, C::VEC>* ring;               // [NP]
+    const Slot<F, C::VEC, C::NH>* ring;        // [NP]
     const ASlot<F, C::VEC, C::NA>* aux;        // [NP]
     int k, lane, cpr;                          // plane, lane, lanes per row
     bool wr;                                   // always true: packed lanes lie in their row
@@ -491,14 +575,14 @@ __device__ __forceinline__ void packed_tile(typename C::F* lds, int nx, int ny, 
         const bool ok = want && k >= 0 && k < ns;
         return make_rsrc(base + (long long)(ok ? k : 0) * sz, ok ? plane_bytes : 0u);
     };
-    auto load = [&](int k, Slot<F, C::VEC>& s, bool want) {
+    auto load = [&](int k, Slot<F, C::VEC, C::NH>& s, bool want) {
         bload<T, C::VEC>(plane_rsrc(op.src, k, want), off_v, s.v);
     };
     auto load_aux = [&](int k, ASlot<F, C::VEC, C::NA>& s, bool want) {
 #pragma unroll
         for (int a = 0; a < C::NAUX; a++) bload<T, C::VEC>(plane_rsrc(op.side[a], k, want), off_a, s.a[a]);
     };
-    auto stage = [&](int k, const Slot<F, C::VEC>& s) {
+    auto stage = [&](int k, const Slot<F, C::VEC, C::NH>& s) {
         if (lane_ok) {
             F* L = L0 + (k % C::NLS) * C::VEC * NL + lane;
 #pragma unroll

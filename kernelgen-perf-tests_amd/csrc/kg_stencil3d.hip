@@ -314,19 +314,25 @@ struct LapgsrbOp {
                                         const bool (&xm)[C::VEC]) const
     {
         double r[C::VEC];
+        // same-row x-neighbours (planes k-1 .. k+1): from registers in the
+        // x-shift mode, else from LDS
+        auto X = [&](int v, int dx, int dz) {
+            if constexpr (C::XS) return p.xat(v, dx, dz);
+            else return p.at(v, dx, 0, dz);
+        };
 #pragma unroll
         for (int v = 0; v < C::VEC; v++)
             r[v] = c0 * p.col(v, 0) +
                 c1 * (
-                    p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                    X(v, 1, 0) + X(v, -1, 0) +
                     p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
                     p.col(v, 1) + p.col(v, -1)) +
                 c2 * (
-                    p.at(v, 1, 1, 0) + p.at(v, 1, 0, 1) + p.at(v, 1, 0, -1) + p.at(v, 1, -1, 0) +
+                    p.at(v, 1, 1, 0) + X(v, 1, 1) + X(v, 1, -1) + p.at(v, 1, -1, 0) +
                     p.at(v, 0, 1, 1) + p.at(v, 0, 1, -1) + p.at(v, 0, -1, 1) + p.at(v, 0, -1, -1) +
-                    p.at(v, -1, 1, 0) + p.at(v, -1, 0, 1) + p.at(v, -1, 0, -1) + p.at(v, -1, -1, 0)) +
+                    p.at(v, -1, 1, 0) + X(v, -1, 1) + X(v, -1, -1) + p.at(v, -1, -1, 0)) +
                 c3 * (
-                    p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
+                    X(v, 2, 0) + X(v, -2, 0) +
                     p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
                     p.col(v, 2) + p.col(v, -2));
         kgp::store<C::VEC>(dst, ob, x, r, xm);
@@ -365,8 +371,19 @@ template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0
 #endif
 template <int V> using GradCfg = kgp::Cfg<V, KG_GRAD_W, 1, 0, 0, 1, 1, 2, 0, 8>;
 template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
+// lapgsrb: 16-byte lanes (even nx, 16-byte aligned fields) take every
+// same-row x-neighbour, in planes k-1 .. k+1, from the neighbouring lanes'
+// registers by DPP shifts (x-shift mode), which leaves LDS only the 12
+// y-offset reads of the 25-point body (20 before: LDS busy 56 -> 37 % of CU
+// cycles, LDS waits 19 -> 7 %).  Sweep at 512x256x256 (30-40 back-to-back
+// launches, means): W=16/D=2 with z-chunks lengthened to about 2 workgroups
+// per CU 120 us (W=12 134, W=8 137; D=3 134-170; fixed 32-plane chunks
+// 129-137, 4 workgroups per CU 131-139); the same run timed the previous
+// shape (8-byte lanes, every neighbour from LDS, W=12/D=2) at 136 us and
+// 8-byte lanes in the x-shift mode at 149-178 us (twice the DPP moves per
+// point), so odd or unaligned grids keep the previous shape.
 #ifndef KG_GSRB_W
-#define KG_GSRB_W 12
+#define KG_GSRB_W 16
 #endif
 #ifndef KG_GSRB_D
 #define KG_GSRB_D 2
@@ -377,10 +394,11 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #ifndef KG_GSRB_WGPC
 #define KG_GSRB_WGPC 2
 #endif
+template <int V> using GsrbCfg = kgp::Cfg<V, (V == 2 ? KG_GSRB_W : 12), 2, 1, 1, 2, 2, KG_GSRB_D, 0,
+                                          KG_GSRB_KC, kgp::F64, 64, false, V == 2>;
 #ifndef KG_WAVE_WGPC
 #define KG_WAVE_WGPC 0
 #endif
-template <int V> using GsrbCfg = kgp::Cfg<1, KG_GSRB_W, 2, 1, 1, 2, 2, KG_GSRB_D, 0, KG_GSRB_KC>;
 
 // Launch Op on the plane engine; 16-byte lanes (VEC=2) need an even nx and
 // 16-byte aligned fields (al).
