@@ -374,6 +374,7 @@ int kg_matmul(kg_ctx* ctx, int nx, int ny, int ns, const double* A, const double
 int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const double* w0, const double* w1, double* w2)
 {
     const double t0_ = now();
+#pragma omp parallel for
     for (int k = 0; k < ns; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
@@ -388,6 +389,7 @@ int kg_vecadd(kg_ctx* ctx, int nx, int ny, int ns, const double* w0, const doubl
 int kg_matvec(kg_ctx* ctx, int nx, int ny, const double* A, const double* x, double* y)
 {
     const double t0_ = now();
+#pragma omp parallel for
     for (int j = 0; j < ny; j++) {
         y[j] = 0;
         for (int i = 0; i < nx; i++) y[j] += A[(size_t)i + (size_t)nx * j] * x[i];
@@ -400,6 +402,7 @@ int kg_matvec(kg_ctx* ctx, int nx, int ny, const double* A, const double* x, dou
 int kg_sincos(kg_ctx* ctx, int nx, int ny, int ns, const double* x, const double* y, double* xy)
 {
     const double t0_ = now();
+#pragma omp parallel for
     for (int k = 0; k < ns; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {

@@ -1,6 +1,7 @@
 ##===----------------------------------------------------------------------===##
 ## Per-target fan-out over the tests (same role as the reference's makefile):
 ##   make gcc | make hip         build every test for one target
+##   make gcc-omp                the CPU target with OpenMP (a second baseline)
 ##   make <test>.hip             build one test
 ##   make test.hip               ./benchmark for the hip target
 ##   make benchmark              ./benchmark NX NY NS NT NRUNS hip gcc
@@ -26,7 +27,7 @@ TARGETS = \
 	wave13pt \
 	whispering
 
-.PHONY: all gcc hip lib clean benchmark test.gcc test.hip
+.PHONY: all gcc gcc-omp hip lib clean benchmark test.gcc test.hip
 
 all: gcc hip
 
@@ -35,10 +36,15 @@ lib:
 
 gcc: $(addsuffix .gcc, $(TARGETS))
 
+gcc-omp: $(addsuffix .gcc-omp, $(TARGETS))
+
 hip: lib $(addsuffix .hip, $(TARGETS))
 
 %.gcc:
 	$(SILENT)$(MAKE) -s -C $(subst .gcc,,$@)/gcc
+
+%.gcc-omp:
+	$(SILENT)$(MAKE) -s -C $(subst .gcc-omp,,$@)/gcc-omp
 
 %.hip: lib
 	$(SILENT)$(MAKE) -s -C $(subst .hip,,$@)/hip
@@ -46,10 +52,14 @@ hip: lib $(addsuffix .hip, $(TARGETS))
 %.gcc.clean:
 	$(SILENT)$(MAKE) -s -C $(subst .gcc.clean,,$@)/gcc clean
 
+%.gcc-omp.clean:
+	$(SILENT)$(MAKE) -s -C $(subst .gcc-omp.clean,,$@)/gcc-omp clean
+
 %.hip.clean:
 	$(SILENT)$(MAKE) -s -C $(subst .hip.clean,,$@)/hip clean
 
-clean: $(addsuffix .gcc.clean, $(TARGETS)) $(addsuffix .hip.clean, $(TARGETS))
+clean: $(addsuffix .gcc.clean, $(TARGETS)) $(addsuffix .gcc-omp.clean, $(TARGETS)) \
+	$(addsuffix .hip.clean, $(TARGETS))
 
 test.gcc: gcc
 	$(SILENT)./benchmark $(NX) $(NY) $(NS) $(NT) $(NRUNS) gcc

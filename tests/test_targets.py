@@ -55,6 +55,18 @@ def test_gcc_target_prints_what_the_reference_program_prints(gcc_built, name):
     assert ours.stdout == theirs.stdout
 
 
+@pytest.mark.parametrize("name", ALL)
+def test_gcc_omp_target_prints_what_the_gcc_target_prints(gcc_built, name):
+    """The OpenMP build of the CPU target (a second, multi-core baseline)
+    computes every point as the single-threaded build does."""
+    build(f"{name}.gcc-omp")
+    omp = run(K.ROOT / name / "gcc-omp" / name, args_for(name), {"NO_TIMING": "1",
+                                                                 "OMP_NUM_THREADS": "4"})
+    one = run(K.ROOT / name / "gcc" / name, args_for(name), {"NO_TIMING": "1"})
+    assert omp.returncode == 0 and one.returncode == 0
+    assert omp.stdout == one.stdout
+
+
 @pytest.mark.parametrize("name", ["laplacian", "gameoflife", "matmul", "vecadd"])
 def test_stdout_contract(gcc_built, name):
     out = run(K.ROOT / name / "gcc" / name, args_for(name)).stdout
@@ -86,6 +98,19 @@ def test_benchmark_driver_table(gcc_built):
         float(cells[6])          # t_comp
         float(cells[7])          # t_krn = t_comp / niters
     assert "Data for table >>" in out and "<< Data for table" in out
+
+
+def test_benchmark_driver_gcc_omp_rows(gcc_built):
+    build("gcc-omp")
+    out = subprocess.run([str(K.ROOT / "benchmark"), "34", "18", "12", "2", "1", "gcc-omp"],
+                         capture_output=True, text=True, cwd=K.ROOT, timeout=600).stdout
+    rows = {m.group(1): m.group(0) for m in re.finditer(r"^\|\s+(\w+) \|\s+gcc-omp \|.*$", out, re.M)}
+    assert sorted(rows) == sorted(ALL)
+    for row in rows.values():
+        cells = [c.strip() for c in row.strip("|").split("|")]
+        assert "FAIL" not in cells, row
+        float(cells[7])          # t_krn = t_comp / niters
+    assert "gcc-omp target host:" in out
 
 
 def test_benchmark_usage():
