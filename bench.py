@@ -166,21 +166,39 @@ def gpu_leg(kg, steps: int, warmup: int, rng, do_calibrate=False):
     return wall, per, cal
 
 
-def cpu_leg(reps: int, rng):
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg(reps: int, rng, omp: bool = False):
     """Reference kernel functions (reference gcc flags) on one core, each
-    stencil `reps` times at the full configuration."""
-    lib_path = ROOT / "oracle" / "_ref" / "libref_fast.so"
+    stencil `reps` times at the full configuration.  omp: the same kernels
+    built with -fopenmp (oracle/_ref/libref_omp.so, their `omp parallel for`
+    active) on OMP_NUM_THREADS host threads; jacobi (Fortran) and whispering
+    (whose reference OpenMP loop is a race) are left out of that build."""
+    lib_path = ROOT / "oracle" / "_ref" / ("libref_omp.so" if omp else "libref_fast.so")
     kind = "reference"
     if not lib_path.exists():
+        if omp:
+            return None
         lib_path = ROOT / "oracle" / "liboracle.so"
         kind = "port"
     if not lib_path.exists():
         return None
     lib = C.CDLL(str(lib_path))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if omp else 1
     pool = {}
     total_t = 0.0
     per = {}
     for name, (dims, bpp, coefs, roles, pairs) in STENCILS.items():
+        if omp and name in ("jacobi", "whispering"):
+            continue
         roles = [r for r in roles if r != "e_next"]
         arrs = {}
         for r in roles:
@@ -206,13 +224,17 @@ def cpu_leg(reps: int, rng):
         per[name] = round(N / dt / 1e9, 4)
         total_t += dt
     pool.clear()
+    if not per:
+        return None
     return {
-        "value": round(len(STENCILS) * N / total_t, 1),
+        "value": round(len(per) * N / total_t, 1),
         "unit": "grid-pts/s",
-        "cores": 1,
+        "cores": threads,
         "kind": kind,
-        "sample": f"each of the {len(STENCILS)} stencils run {reps}x at 512x256x256 / 512x65536 "
-                  f"on one host core ({lib_path.name}); per-stencil Gpts/s: {json.dumps(per)}",
+        "cpu": cpu_model(),
+        "sample": f"each of the {len(per)} stencils {sorted(per)} run {reps}x at 512x256x256 / "
+                  f"512x65536 on {threads} host thread(s) ({lib_path.name}); "
+                  f"per-stencil Gpts/s: {json.dumps(per)}",
     }
 
 
@@ -266,7 +288,8 @@ def layers_leg(kg, suite_per: dict) -> dict:
             "loads": [{"offered_load": r["offered_load"],
                        **{m: {k: round(r[m][k], 4) for k in ("p50_s", "p99_s", "gpts_s")}
                           for m in ("continuous", "sequential")},
-                       "peak_active": r["continuous"]["peak_active"]} for r in s["loads"]]}
+                       "peak_active": r["continuous"]["peak_active"],
+                       "checked": r["checked"]} for r in s["loads"]]}
         p = L.run_precision(kg, ctx)
         out["precision_512cubed_1000_steps"] = {
             kind: {"final": {d: {k: float("%.3g" % v) for k, v in p[kind]["checks"][-1][d].items()}
@@ -289,7 +312,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also time a plain 512 MiB device copy (reported as copy_calibration)")
@@ -355,6 +378,9 @@ def main() -> int:
         per["tricubic"]["frac_fp64_issue"] = round(252 * N / t / 39.3e12, 4)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg(args.cpu_reps, rng)
+        omp = cpu_leg(args.cpu_reps, rng, omp=True)
+        if omp:
+            line["cpu_baseline_omp"] = omp
     if not args.no_layers and not args.only:
         line["layers"] = layers_leg(kg, per)
     print(json.dumps(line), flush=True)

@@ -53,14 +53,57 @@ def batch_grids(seed: int = 1, ngrids: int = 4096, lo: int = 16, hi: int = 256):
     return [(str(k), tuple(int(v) for v in e)) for k, e in zip(kinds, edges)]
 
 
+# single-grid entry point of each batch kind, per dtype
+SINGLE = {("laplacian", 0): "laplacian", ("laplacian", 1): "laplacian_f32",
+          ("wave13pt", 0): "wave13pt", ("wave13pt", 1): "wave13pt_f32",
+          ("gaussblur3", 0): "gaussblur3_f64", ("gaussblur3", 1): "gaussblur3_f32"}
+
+
+def check_sample(grids, seed: int, n: int = 24) -> list[int]:
+    """Ids of n grids spread over the kinds (each kind's grids drawn from
+    seed), the sample whose batched results are checked."""
+    rng = np.random.default_rng(seed + 99)
+    ids = []
+    for kind in ("laplacian", "wave13pt", "gaussblur3"):
+        pool = [i for i, (k, _) in enumerate(grids) if k == kind]
+        ids += [int(i) for i in rng.choice(pool, min(len(pool), n // 3), replace=False)]
+    return sorted(ids)
+
+
+def single_grid_replay(kg, ctx, kind: str, dtype: int, dims, levels, steps: int) -> np.ndarray:
+    """The grid stepped `steps` times by its kind's single-grid entry point
+    from the given initial levels; returns the newest level (ns, ny, nx)."""
+    fn = SINGLE[(kind, dtype)]
+    bufs = [ctx.array(a.ravel()) for a in levels]
+    for _ in range(steps):
+        ctx.call(fn, *dims, *COEFS[kind], *bufs)
+        bufs = bufs[1:] + bufs[:1]
+    ctx.sync()
+    # after the rotation the level written last is the one before the input
+    # position: laplacian/gaussblur3 [in, out] -> [out, in]: bufs[0];
+    # wave13pt [w0, w1, w2] -> [w1, w2, w0]: bufs[1]
+    newest = bufs[0] if kind != "wave13pt" else bufs[1]
+    out = newest.download().reshape(dims[2], dims[1], dims[0])
+    for b in bufs:
+        b.free()
+    return out
+
+
 def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int = 4096,
-              tune_reps: int = 2, seq_steps: int = 3) -> dict:
+              tune_reps: int = 2, seq_steps: int = 3, check: bool = True) -> dict:
+    """check: after the timed steps, a sample of grids (check_sample) is
+    compared bit for bit with the same grids stepped as often by the
+    single-grid entry points from the same initial levels."""
     grids = batch_grids(seed, ngrids)
     b = kg.Batch(ctx)
+    checked = None
     try:
         for kind, dims in grids:
             b.add(kind, dtype, dims, COEFS[kind])
         b.fill(seed)
+        sample = check_sample(grids, seed) if check else []
+        initial = {g: [b.download(g, l) for l in range(3 if grids[g][0] == "wave13pt" else 2)]
+                   for g in sample}
         t0 = time.perf_counter()
         ntuned = b.tune(tune_reps)
         t_tune = time.perf_counter() - t0
@@ -77,6 +120,17 @@ def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int 
         b.step_sequential(seq_steps)
         ctx.sync()
         t_seq = time.perf_counter() - t0
+        if check:
+            bad = []
+            for g in sample:
+                kind, dims = grids[g]
+                got = b.download(g)
+                want = single_grid_replay(kg, ctx, kind, dtype, dims, initial[g], b.steps_done(g))
+                if not np.array_equal(got.view(np.uint8), want.view(np.uint8)):
+                    bad.append(g)
+            checked = {"grids": len(sample), "steps": b.steps_done(sample[0]) if sample else 0,
+                       "against": "single-grid entry points, same initial levels",
+                       "bit_identical": not bad, "mismatched": bad}
     finally:
         b.close()
     pts = info["points_per_step"]
@@ -91,6 +145,7 @@ def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int 
         "frac_hbm": info["bytes_per_step"] * steps / t / 1e9 / HBM_PEAK_GBS,
         "sequential_ms_per_step": 1e3 * t_seq / seq_steps,
         "sequential_gpts_s": pts * seq_steps / t_seq / 1e9,
+        "checked": checked,
     }
 
 
@@ -128,13 +183,19 @@ def run_serve(kg, ctx, njobs: int = 120, loads=(0.5, 0.8, 1.0, 1.5), seed: int =
     for load in loads:
         jobs = serve_jobs(njobs, load * cap, seed)
         row = {"offered_load": load, "rate_jobs_s": load * cap}
+        sums = {}
         for name, mode in (("continuous", kg.SERVE_CONTINUOUS), ("sequential", kg.SERVE_SEQUENTIAL)):
             res, st, _ = kg.serve(ctx, jobs, budget, max_active, mode)
             ok = [r for r in res if r["status"] == 0]
+            sums[name] = [(r["status"], r["steps_run"], r["checksum"]) for r in res]
             row[name] = {"p50_s": st["p50"], "p99_s": st["p99"], "mean_s": st["mean_latency"],
                          "gpts_s": st["points"] / st["wall"] / 1e9, "wall_s": st["wall"],
                          "peak_active": st["peak_active"], "peak_bytes": st["peak_bytes"],
                          "completed": len(ok)}
+        # every job completed, and each job's final field (its FP64 sum and
+        # sum of squares) is the same in the mix as when run alone
+        row["checked"] = {"all_completed": all(row[m]["completed"] == njobs for m in sums),
+                          "same_results_in_the_mix": sums["continuous"] == sums["sequential"]}
         out["loads"].append(row)
     return out
 

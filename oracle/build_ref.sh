@@ -15,6 +15,10 @@
 #                        flush-to-zero.
 #   _ref/bin/<test>      the reference's gcc-target programs (main included),
 #                        reference flags, for its printed means.
+#   _ref/libref_omp.so   the C kernels with the reference flags plus -fopenmp
+#                        (their `#pragma omp parallel for` active): the
+#                        multi-core CPU baseline of bench.py.  The Fortran
+#                        kernels are left out (a second OpenMP runtime).
 #
 # Each C test file is compiled with -Dmain=kgref_main_<test> so that its kernel
 # function can be called directly; nothing is copied or stubbed.  The Fortran
@@ -42,7 +46,7 @@ CTESTS="laplacian divergence gradient wave13pt uxx1 tricubic tricubic2 lapgsrb g
 FTESTS="jacobi matmul sincos"
 
 rm -rf "$OUT"
-mkdir -p "$OUT/ieee" "$OUT/fast" "$OUT/bin"
+mkdir -p "$OUT/ieee" "$OUT/fast" "$OUT/omp" "$OUT/bin"
 
 COMMON="-std=c99 -D_GNU_SOURCE -Dreal=double -fPIC -w"
 IEEE="-O2 -fno-fast-math -ffp-contract=off"
@@ -57,6 +61,9 @@ for t in $CTESTS; do
     gcc $COMMON $IEEE -I"$REF/$t" -Dmain=kgref_main_$t -c "$src" -o "$OUT/ieee/$t.o"
     if [ "$t" = whispering ]; then fl="$FAST_O0"; else fl="$FAST"; fi
     gcc $COMMON $fl -I"$REF/$t" -Dmain=kgref_main_$t -c "$src" -o "$OUT/fast/$t.o"
+    if [ "$t" != whispering ]; then
+        gcc $COMMON $FAST -fopenmp -I"$REF/$t" -Dmain=kgref_main_$t -c "$src" -o "$OUT/omp/$t.o"
+    fi
     # The reference gcc-target program itself (timing.c from the same test dir).
     gcc $COMMON $fl -I"$REF/$t" "$src" "$REF/$t/timing.c" -o "$OUT/bin/$t" -lrt
 done
@@ -71,5 +78,6 @@ done
 
 gcc -shared -o "$OUT/libref_ieee.so" "$OUT"/ieee/*.o -lrt -lm
 gcc -shared -o "$OUT/libref_fast.so" $(ls "$OUT"/fast/*.o | grep -v _prog.o) -lrt -lm
-rm -rf "$OUT/ieee" "$OUT/fast"
+gcc -shared -fopenmp -o "$OUT/libref_omp.so" "$OUT"/omp/*.o "$OUT/fast/timing.o" -lrt -lm
+rm -rf "$OUT/ieee" "$OUT/fast" "$OUT/omp"
 echo "build_ref.sh: built $OUT (march=$MARCH)"
