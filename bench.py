@@ -238,6 +238,36 @@ def cpu_leg(reps: int, rng, omp: bool = False):
     }
 
 
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def matmul_leg(kg, reps: int = 20) -> dict:
+    """matmul at the benchmark configuration (C(512,256) += A(512,256) B(256,256),
+    FP64 on the matrix cores): mean launch time, achieved TFLOP/s and its
+    fraction of the FP64 matrix peak."""
+    nx, ny, ns = NX, NY, NS
+    rng = np.random.default_rng(12)
+    ctx = kg.Context(0)
+    A = ctx.array(rng.uniform(-1, 1, nx * ny))
+    B = ctx.array(rng.uniform(-1, 1, ny * ns))
+    Cm = ctx.array(np.zeros(nx * ns))
+    for _ in range(3):
+        ctx.call("matmul", nx, ny, ns, A, B, Cm)
+    ctx.sync()
+    ctx.drain()
+    for _ in range(reps):
+        ctx.call("matmul", nx, ny, ns, A, B, Cm)
+    t = float(np.mean(ctx.drain()))
+    regs = ctx.regcount("matmul_")
+    for a in (A, B, Cm):
+        a.free()
+    ctx.close()
+    flop = 2.0 * nx * ny * ns
+    return {"dims": [nx, ny, ns], "flop_per_launch": flop, "us": round(t * 1e6, 2),
+            "tflops": round(flop / t / 1e12, 3),
+            "frac_fp64_mfma_peak": round(flop / t / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4), "regs": regs}
+
+
 def load_layers():
     spec = importlib.util.spec_from_file_location("kglayers", PKG / "kglayers.py")
     mod = importlib.util.module_from_spec(spec)
@@ -369,6 +399,8 @@ def main() -> int:
     }
     if cal:
         line["copy_calibration"] = cal
+    if not args.only:
+        line["matmul"] = matmul_leg(kg)
     if "tricubic" in per:
         # tricubic is bound by FP64 issue: 252 multiplies and adds per point
         # (no contraction), against 39.3 T non-FMA FP64 ops/s (78.6 TFLOP/s
