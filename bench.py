@@ -305,6 +305,7 @@ def layers_leg(kg, suite_per: dict) -> dict:
         plain = suite_per.get("laplacian", {}).get("gpts_s")
         for name, dt in (("batch_f64", kg.F64), ("batch_f32", kg.F32)):
             r = L.run_batch(kg, ctx, dt)
+            r.pop("shapes", None)
             out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
         if plain:
             out["batch_f32_vs_plain_fp64_laplacian_512x256x256"] = round(

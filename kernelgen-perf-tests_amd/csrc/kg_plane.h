@@ -697,16 +697,53 @@ void rows2_kernel(int nx, int ny, int ntx, int nty, Op op)
 // planes, tiles enumerated x fastest, then y, then z-chunk, in tile order ORD
 // (kg_tile_order: hardware, XCD-contiguous or chunks of ORD tile rows).
 // z-interior [ZB0, ns-ZB1), by default the same as the xy-interior.
+// ORD < 0: tiles enumerated z-chunk fastest, then x, then y, and dealt to
+// the XCDs in runs of -ORD whole z-columns (kg_xcd_chunk): the z-chunks of a
+// tile column, which share the planes at their ends, run on one XCD.
+// ORD == KG_ORD_REGION: each XCD owns a fixed region of the xy-tiles, an
+// eighth of them in x-fastest order, and walks it z-chunk by z-chunk; the
+// x-, y- and z-neighbours of a tile (which read its halo lines and end
+// planes) then run on the same XCD, most of them at the same time.  The
+// launch has 8 * region_tiles(ntx*nty) * nkc workgroups (region_blocks);
+// slots past the last tile of a region return at once.
+constexpr int KG_ORD_REGION = 100;
+__host__ __device__ inline unsigned region_tiles(unsigned tiles_per_layer)
+{
+    return (tiles_per_layer + KG_NUM_XCD - 1) / KG_NUM_XCD;
+}
+__host__ __device__ inline unsigned region_blocks(unsigned tiles_per_layer, unsigned nkc)
+{
+    return KG_NUM_XCD * region_tiles(tiles_per_layer) * nkc;
+}
+
 template <class C, int LO, int HI, class Op, int ZB0 = LO, int ZB1 = HI, int ORD = 1>
 __global__ __launch_bounds__(64 * C::W)
 void plane_kernel(int nx, int ny, int ns, int ntx, int nty, int kc, Op op)
 {
     __shared__ typename C::F lds[C::LDS_BYTES / sizeof(typename C::F)];
-    unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
-    const int bx = (int)(b % (unsigned)ntx);
-    b /= (unsigned)ntx;
-    const int by = (int)(b % (unsigned)nty);
-    const int bz = (int)(b / (unsigned)nty);
+    int bx, by, bz;
+    if constexpr (ORD == KG_ORD_REGION) {
+        const unsigned L = (unsigned)ntx * (unsigned)nty, R = region_tiles(L);
+        const unsigned x = blockIdx.x % KG_NUM_XCD, s = blockIdx.x / KG_NUM_XCD;
+        const unsigned t = x * R + s % R;   // tile of the layer
+        if (t >= L) return;                 // past the last tile: the whole workgroup leaves
+        bz = (int)(s / R);
+        bx = (int)(t % (unsigned)ntx);
+        by = (int)(t / (unsigned)ntx);
+    } else if constexpr (ORD < 0) {
+        const unsigned nkc = gridDim.x / ((unsigned)ntx * (unsigned)nty);
+        unsigned b = kg_xcd_chunk(blockIdx.x, gridDim.x, nkc * (unsigned)(-ORD));
+        bz = (int)(b % nkc);
+        b /= nkc;
+        bx = (int)(b % (unsigned)ntx);
+        by = (int)(b / (unsigned)ntx);
+    } else {
+        unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
+        bx = (int)(b % (unsigned)ntx);
+        b /= (unsigned)ntx;
+        by = (int)(b % (unsigned)nty);
+        bz = (int)(b / (unsigned)nty);
+    }
     plane_tile<C, LO, HI, ZB0, ZB1, Op>(lds, nx, ny, ns, nx, (long long)nx * ny, bx, by, bz,
                                         kc, op);
 }

@@ -435,8 +435,10 @@ static int launch_plane(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
         auto kern = kgp::plane_kernel<C, LO, HI, Op, LO, HI, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
+        const unsigned blocks = ORD == kgp::KG_ORD_REGION
+            ? kgp::region_blocks((unsigned)g.ntx * (unsigned)g.nty, (unsigned)g.nkc) : g.blocks;
         if (!g.empty)
-            hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(64 * C::W), smem, ctx->stream,
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * C::W), smem, ctx->stream,
                                nx, ny, ns, g.ntx, g.nty, kc, op);
         return kg_launch_end(ctx);
     };

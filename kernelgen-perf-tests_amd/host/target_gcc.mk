@@ -23,14 +23,27 @@ $(TEST): ../$(TEST).c $(HOSTDIR)/kg_host.c $(HOSTDIR)/kg_host.h kg_cpu.o kg_cpu_
 	$(SILENT)$(MAINCC) $(OMPFLAGS) ../$(TEST).c $(HOSTDIR)/kg_host.c kg_cpu.o kg_cpu_whispering.o -o $@ -lrt -lm
 
 clean:
-	$(SILENT)rm -f $(TEST) *.o
+	$(SILENT)rm -f $(TEST) $(TEST)_check *.o
 
 ifeq ($(DIMS), 2)
-test: $(TEST)
-	$(SILENT)./$< $(NX) $$(($(NY) * $(NS))) $(NT)
+RUNARGS = $(NX) $$(($(NY) * $(NS))) $(NT)
 else
-test: $(TEST)
-	$(SILENT)./$< $(NX) $(NY) $(NS) $(NT)
+RUNARGS = $(NX) $(NY) $(NS) $(NT)
 endif
 
-.PHONY: all clean test
+test: $(TEST)
+	$(SILENT)./$< $(RUNARGS)
+
+# check: the program with the CPU backend, every C file built with
+# AddressSanitizer and UndefinedBehaviorSanitizer, run once (the reference's
+# cuda-memcheck rule has no CPU counterpart; this is the host-side one).
+SANFLAGS = -g -O1 -fno-omit-frame-pointer -fsanitize=address,undefined -fno-sanitize-recover=undefined
+$(TEST)_check: ../$(TEST).c $(HOSTDIR)/kg_host.c $(HOSTDIR)/kg_host.h $(HOSTDIR)/kg_cpu.c \
+		$(HOSTDIR)/kg_cpu_whispering.c ../../include/kg_stencil.h
+	$(SILENT)gcc -D_GNU_SOURCE -std=c99 $(SANFLAGS) $(OMPFLAGS) ../$(TEST).c $(HOSTDIR)/kg_host.c \
+		$(HOSTDIR)/kg_cpu.c $(HOSTDIR)/kg_cpu_whispering.c -o $@ -lrt -lm
+
+check: $(TEST)_check
+	$(SILENT)ASAN_OPTIONS=detect_leaks=1 UBSAN_OPTIONS=print_stacktrace=1 ./$< $(RUNARGS)
+
+.PHONY: all clean test check

@@ -90,10 +90,14 @@ def single_grid_replay(kg, ctx, kind: str, dtype: int, dims, levels, steps: int)
 
 
 def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int = 4096,
-              tune_reps: int = 2, seq_steps: int = 3, check: bool = True) -> dict:
+              tune_reps: int = 2, seq_steps: int = 3, check: bool = True,
+              shapes: dict | None = None) -> dict:
     """check: after the timed steps, a sample of grids (check_sample) is
     compared bit for bit with the same grids stepped as often by the
-    single-grid entry points from the same initial levels."""
+    single-grid entry points from the same initial levels.  shapes: tile
+    shapes {grid id: (lanes per row, planes per tile)} from an earlier tuned
+    run (result key "shapes"), applied instead of autotuning -- profiling
+    runs use it so that their traces hold the stepping launches only."""
     grids = batch_grids(seed, ngrids)
     b = kg.Batch(ctx)
     checked = None
@@ -105,8 +109,14 @@ def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int 
         initial = {g: [b.download(g, l) for l in range(3 if grids[g][0] == "wave13pt" else 2)]
                    for g in sample}
         t0 = time.perf_counter()
-        ntuned = b.tune(tune_reps)
+        if shapes is None:
+            ntuned = b.tune(tune_reps)
+        else:
+            ntuned = 0
+            for g, (lpr, ppt) in shapes.items():
+                b.set_shape(int(g), int(lpr), int(ppt))
         t_tune = time.perf_counter() - t0
+        used = {g: b.shape(g) for g in range(len(grids))}
         info = b.info()
         reserved, in_use = b.memory()
         b.step(1, graph=False)                 # warm
@@ -117,7 +127,8 @@ def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int 
         ctx.sync()
         t = time.perf_counter() - t0
         t0 = time.perf_counter()
-        b.step_sequential(seq_steps)
+        if seq_steps > 0:
+            b.step_sequential(seq_steps)
         ctx.sync()
         t_seq = time.perf_counter() - t0
         if check:
@@ -143,9 +154,10 @@ def run_batch(kg, ctx, dtype: int, steps: int = 100, seed: int = 1, ngrids: int 
         "gpts_s": pts * steps / t / 1e9,
         "gbs": info["bytes_per_step"] * steps / t / 1e9,
         "frac_hbm": info["bytes_per_step"] * steps / t / 1e9 / HBM_PEAK_GBS,
-        "sequential_ms_per_step": 1e3 * t_seq / seq_steps,
-        "sequential_gpts_s": pts * seq_steps / t_seq / 1e9,
+        "sequential_ms_per_step": 1e3 * t_seq / seq_steps if seq_steps > 0 else None,
+        "sequential_gpts_s": pts * seq_steps / t_seq / 1e9 if seq_steps > 0 else None,
         "checked": checked,
+        "shapes": used,
     }
 
 
@@ -312,13 +324,24 @@ def _main(argv=None) -> int:
     ap.add_argument("layer", choices=["batch", "serve", "precision", "poisson"])
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--save-shapes", help="batch: write the tuned tile shapes to this JSON file")
+    ap.add_argument("--load-shapes", help="batch: apply these tile shapes instead of autotuning")
+    ap.add_argument("--no-check", action="store_true", help="batch: skip the sampled result check")
+    ap.add_argument("--seq-steps", type=int, default=3,
+                    help="batch: steps of the one-launch-per-grid comparison (0: none)")
     args = ap.parse_args(argv)
     spec = importlib.util.spec_from_file_location("kgstencil", Path(__file__).with_name("kgstencil.py"))
     kg = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(kg)
     with kg.Context(0) as ctx:
         if args.layer == "batch":
-            r = run_batch(kg, ctx, kg.F64 if args.dtype == "f64" else kg.F32, steps=args.steps)
+            shapes = json.load(open(args.load_shapes)) if args.load_shapes else None
+            r = run_batch(kg, ctx, kg.F64 if args.dtype == "f64" else kg.F32, steps=args.steps,
+                          check=not args.no_check, shapes=shapes, seq_steps=args.seq_steps)
+            if args.save_shapes:
+                with open(args.save_shapes, "w") as fh:
+                    json.dump(r["shapes"], fh)
+            r.pop("shapes")
         elif args.layer == "serve":
             r = run_serve(kg, ctx)
         elif args.layer == "precision":

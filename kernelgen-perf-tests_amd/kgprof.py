@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import collections
+import csv
 import json
 import re
 import sqlite3
@@ -45,17 +46,38 @@ def short_name(full: str) -> str:
     return full.split("(")[0][:40]
 
 
-def durations(db: str) -> dict:
+def durations(path: str) -> dict:
+    """Kernel durations (ns) per kernel from a rocpd database or a
+    *_kernel_trace.csv file."""
     out = collections.defaultdict(list)
-    con = sqlite3.connect(db)
+    if path.endswith(".csv"):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                out[short_name(r["Kernel_Name"])].append(
+                    int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        return out
+    con = sqlite3.connect(path)
     for name, dur in con.execute("select name, duration from kernels"):
         out[short_name(name)].append(dur)
     return out
 
 
-def counters(db: str, counter: str) -> dict:
+def counters(path: str, counter: str) -> dict:
+    """Per-dispatch values of a counter (summed over its dimensions) per
+    kernel, from a rocpd database or a *_counter_collection.csv file."""
     out = collections.defaultdict(list)
-    con = sqlite3.connect(db)
+    if path.endswith(".csv"):
+        per = collections.defaultdict(float)
+        names = {}
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] == counter:
+                    per[r["Dispatch_Id"]] += float(r["Counter_Value"])
+                    names[r["Dispatch_Id"]] = r["Kernel_Name"]
+        for d, v in per.items():
+            out[short_name(names[d])].append(v)
+        return out
+    con = sqlite3.connect(path)
     for name, val in con.execute(
             "select kernel_name, value from counters_collection where counter_name = ?", (counter,)):
         out[short_name(name)].append(val)
@@ -63,6 +85,7 @@ def counters(db: str, counter: str) -> dict:
 
 
 def main(argv=None) -> int:
+    global N
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
     ap.add_argument("--fetch")
@@ -70,9 +93,23 @@ def main(argv=None) -> int:
     ap.add_argument("--cmd", default="")
     ap.add_argument("--copy-bytes", type=float, default=2 * N * 8,
                     help="bytes the calibration copy reads (= writes) per launch")
+    ap.add_argument("--scale", type=float, nargs=2, default=None, metavar=("FETCH", "WRITE"),
+                    help="counter scales when the trace has no calibration copy "
+                         "(measured on MI355X with stream_copy: 2.0 1.0)")
+    ap.add_argument("--nx", type=int, default=None, help="grid points per launch (default 512x256x256)")
+    ap.add_argument("--match", help="report only kernels whose full name contains this text, "
+                    "as one row named by it (e.g. a batch's stepping kernel)")
+    ap.add_argument("--alg-bytes", type=float, help="algorithmic bytes per launch of --match")
     ap.add_argument("--out")
     ap.add_argument("--traffic-json")
     a = ap.parse_args(argv)
+    if a.match:
+        global short_name
+        plain = short_name
+        short_name = lambda full: a.match if a.match in full else "other:" + plain(full)
+        BYTES_PER_POINT.clear()
+    if a.nx:
+        N = a.nx
 
     dur = durations(a.trace)
     fetch = counters(a.fetch, "FETCH_SIZE") if a.fetch else {}
@@ -82,11 +119,16 @@ def main(argv=None) -> int:
         mean = lambda v: sum(v) / len(v)
         fscale = a.copy_bytes / (mean(fetch["stream_copy"]) * 1024)
         wscale = a.copy_bytes / (mean(write["stream_copy"]) * 1024)
+    elif a.scale and fetch and write:
+        fscale, wscale = a.scale
 
     lines = []
     if a.cmd:
         lines.append(f"command: {a.cmd}")
-    if fscale:
+    if fscale and a.scale and not fetch.get("stream_copy"):
+        lines.append(f"counter scales (measured with the calibration copy on MI355X): "
+                     f"FETCH_SIZE x {fscale:.4f}, WRITE_SIZE x {wscale:.4f}")
+    elif fscale:
         lines.append(f"counter calibration (stream_copy, {a.copy_bytes / 2**20:.0f} MiB read + written): "
                      f"FETCH_SIZE x {fscale:.4f}, WRITE_SIZE x {wscale:.4f}")
     lines.append(f"{'kernel':12s} {'launches':>8s} {'mean us':>9s} {'alg MB':>8s} {'alg GB/s':>9s} "
@@ -95,8 +137,12 @@ def main(argv=None) -> int:
     for k in sorted(dur, key=lambda k: -sum(dur[k])):
         d = dur[k]
         us = sum(d) / len(d) / 1e3
+        if a.match and k.startswith("other:"):
+            continue
         bpp = BYTES_PER_POINT.get(k)
         alg = bpp * N if bpp else (2 * a.copy_bytes if k == "stream_copy" else None)
+        if a.match and k == a.match:
+            alg = a.alg_bytes
         rd = wr = ratio = None
         if fscale and k in fetch and k in write:
             rd = sum(fetch[k]) / len(fetch[k]) * 1024 * fscale

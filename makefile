@@ -2,6 +2,8 @@
 ## Per-target fan-out over the tests (same role as the reference's makefile):
 ##   make gcc | make hip         build every test for one target
 ##   make gcc-omp                the CPU target with OpenMP (a second baseline)
+##   make check.gcc | check.hip  every test once under ASan + UBSan (host code)
+##   make roofline.hip           rocprofv3 traffic and bandwidth of every hip test
 ##   make <test>.hip             build one test
 ##   make test.hip               ./benchmark for the hip target
 ##   make benchmark              ./benchmark NX NY NS NT NRUNS hip gcc
@@ -27,7 +29,7 @@ TARGETS = \
 	wave13pt \
 	whispering
 
-.PHONY: all gcc gcc-omp hip lib clean benchmark test.gcc test.hip
+.PHONY: all gcc gcc-omp hip lib clean benchmark test.gcc test.hip check.gcc check.hip roofline.hip
 
 all: gcc hip
 
@@ -48,6 +50,21 @@ hip: lib $(addsuffix .hip, $(TARGETS))
 
 %.hip: lib
 	$(SILENT)$(MAKE) -s -C $(subst .hip,,$@)/hip
+
+check.gcc: $(addsuffix .gcc.check, $(TARGETS))
+
+check.hip: lib $(addsuffix .hip.check, $(TARGETS))
+
+roofline.hip: lib $(addsuffix .hip.roofline, $(TARGETS))
+
+%.gcc.check:
+	$(SILENT)$(MAKE) -s -C $(subst .gcc.check,,$@)/gcc check
+
+%.hip.check: lib
+	$(SILENT)$(MAKE) -s -C $(subst .hip.check,,$@)/hip check
+
+%.hip.roofline: lib
+	$(SILENT)$(MAKE) -s -C $(subst .hip.roofline,,$@)/hip roofline
 
 %.gcc.clean:
 	$(SILENT)$(MAKE) -s -C $(subst .gcc.clean,,$@)/gcc clean

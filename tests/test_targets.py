@@ -113,6 +113,18 @@ def test_benchmark_driver_gcc_omp_rows(gcc_built):
     assert "gcc-omp target host:" in out
 
 
+@pytest.mark.parametrize("name", ["laplacian", "whispering", "tricubic", "matvec"])
+def test_gcc_check_rule_runs_clean_under_sanitizers(gcc_built, name):
+    """`make check` builds the program and its CPU backend with ASan and UBSan
+    and runs it; any report fails the run."""
+    dims = ["NX=34", "NY=18", "NS=12", "NT=2"]
+    r = subprocess.run(["make", "-s", "-C", str(K.ROOT / name / "gcc"), "check", *dims],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
+    assert "final mean" in r.stdout
+
+
 def test_benchmark_usage():
     out = subprocess.run([str(K.ROOT / "benchmark")], capture_output=True, text=True).stdout
     assert out.startswith("Usage: benchmark <nx> <ny> <ns> <niters> <nruns> <target>")
