@@ -295,6 +295,25 @@ def life_leg(kg, ctx, n: int = 8192, reps: int = 20) -> dict:
             "gbs": round(2 * n * n / t / 1e9, 1), "frac_hbm": round(2 * n * n / t / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def plain_rate(kg, ctx, fn: str, coefs, nlev: int, reps: int = 10) -> float:
+    """Grid points per second of one single-grid FP64 entry point at
+    512x256x256 (levels rotated between launches)."""
+    rng = np.random.default_rng(3)
+    bufs = [ctx.array(rng.uniform(-1, 1, N)) for _ in range(nlev)]
+    for _ in range(2):
+        ctx.call(fn, *DIMS3, *coefs, *bufs)
+        bufs = bufs[1:] + bufs[:1]
+    ctx.sync()
+    ctx.drain()
+    for _ in range(reps):
+        ctx.call(fn, *DIMS3, *coefs, *bufs)
+        bufs = bufs[1:] + bufs[:1]
+    t = float(np.mean(ctx.drain()))
+    for b in bufs:
+        b.free()
+    return N / t
+
+
 def layers_leg(kg, suite_per: dict) -> dict:
     """The layers beside the drop-in kernels, summarised for the JSON line."""
     L = load_layers()
@@ -312,6 +331,22 @@ def layers_leg(kg, suite_per: dict) -> dict:
                 out["batch_f32"]["gpts_s"] / plain, 3)
             out["batch_f32_vs_sequential_fp64_launches"] = round(
                 out["batch_f32"]["gpts_s"] / out["batch_f64"]["sequential_gpts_s"], 2)
+            # The same mix of kinds done by the plain FP64 single-grid kernels
+            # at 512x256x256: the batch's points of each kind at that kind's
+            # plain rate.
+            rates = {"laplacian": plain * 1e9,
+                     "wave13pt": suite_per["wave13pt"]["gpts_s"] * 1e9,
+                     "gaussblur3": plain_rate(kg, ctx, "gaussblur3_f64", L.GAUSS, 2)}
+            pts = {k: 0 for k in rates}
+            for kind, (nx, ny, ns) in L.batch_grids(1, 4096):
+                pts[kind] += nx * ny * ns
+            t_plain = sum(pts[k] / rates[k] for k in pts)
+            mix = sum(pts.values()) / t_plain
+            out["plain_fp64_same_kinds_gpts_s"] = {
+                "mix": round(mix / 1e9, 2), "points_share": {k: round(v / sum(pts.values()), 4)
+                                                              for k, v in pts.items()},
+                **{k: round(v / 1e9, 2) for k, v in rates.items()}}
+            out["batch_f32_vs_plain_fp64_same_kinds"] = round(out["batch_f32"]["gpts_s"] * 1e9 / mix, 3)
         s = L.run_serve(kg, ctx)
         out["serve"] = {
             "jobs": s["jobs"], "budget_bytes": s["budget_bytes"],

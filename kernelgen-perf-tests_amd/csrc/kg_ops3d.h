@@ -29,7 +29,7 @@ struct LaplacianOp {
 #pragma unroll
         for (int v = 0; v < C::VEC; v++)
             r[v] = alpha * p.col(v, 0) + beta * (
-                p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                kgp::xrow<C>(p, v, 1) + kgp::xrow<C>(p, v, -1) +
                 p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
                 p.col(v, 1) + p.col(v, -1));
         p.put(dst, ob, x, r, xm);
@@ -55,11 +55,11 @@ struct Wave13ptOp {
         for (int v = 0; v < C::VEC; v++)
             r[v] = m0 * p.col(v, 0) - p.side(0, v) +
                 m1 * (
-                    p.at(v, 1, 0, 0) + p.at(v, -1, 0, 0) +
+                    kgp::xrow<C>(p, v, 1) + kgp::xrow<C>(p, v, -1) +
                     p.at(v, 0, 1, 0) + p.at(v, 0, -1, 0) +
                     p.col(v, 1) + p.col(v, -1)) +
                 m2 * (
-                    p.at(v, 2, 0, 0) + p.at(v, -2, 0, 0) +
+                    kgp::xrow<C>(p, v, 2) + kgp::xrow<C>(p, v, -2) +
                     p.at(v, 0, 2, 0) + p.at(v, 0, -2, 0) +
                     p.col(v, 2) + p.col(v, -2));
         p.put(dst, ob, x, r, xm);
@@ -198,9 +198,11 @@ template <> struct Shape<Jacobi2Op> { static constexpr int H = 1, LL = 0, LR = 0
 template <> struct Shape<Life2Op> { static constexpr int H = 1, LL = 0, LR = 0, ZL = 0, ZR = 0, NAUX = 0; };
 
 // Engine configuration of op Op with storage type T, VEC points per lane, W
-// waves, prefetch depth D, KC planes per single-grid tile, LPR lanes per row.
-template <template <class> class Op, class T, int VEC, int W, int D, int KC, int LPR = 64>
+// waves, prefetch depth D, KC planes per single-grid tile, LPR lanes per row,
+// XS the x-shift mode (LPR = 64 only).
+template <template <class> class Op, class T, int VEC, int W, int D, int KC, int LPR = 64,
+          bool XS = false>
 using OpCfg = kgp::Cfg<VEC, W, Shape<Op>::H, Shape<Op>::LL, Shape<Op>::LR, Shape<Op>::ZL,
-                       Shape<Op>::ZR, D, Shape<Op>::NAUX, KC, T, LPR>;
+                       Shape<Op>::ZR, D, Shape<Op>::NAUX, KC, T, LPR, false, XS>;
 
 }  // namespace kgops

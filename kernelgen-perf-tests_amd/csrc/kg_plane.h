@@ -148,6 +148,8 @@ __device__ __forceinline__ void bload(__amdgpu_buffer_rsrc_t r, unsigned off,
 }
 
 // ---------------------------------------------------------------- config
+template <class C, class V>
+__device__ __forceinline__ typename C::F xrow(const V& p, int v, int dx);
 template <int VEC_, int W_, int H_, int LL_, int LR_, int ZL_, int ZR_, int D_, int NAUX_, int KC_,
           class T_ = F64, int LPR_ = 64, bool FULL_ = false, bool XS_ = false>
 struct Cfg {
@@ -318,6 +320,16 @@ struct View {
     }
     __device__ __forceinline__ F side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
 };
+
+// Main field at (x + v + dx, j, k) -- a neighbour along the lane's own row:
+// from registers (in-lane components, DPP shifts across lanes) in the x-shift
+// mode, from LDS otherwise.  Any view type of the engines and the batch.
+template <class C, class V>
+__device__ __forceinline__ typename C::F xrow(const V& p, int v, int dx)
+{
+    if constexpr (C::XS) return p.xat(v, dx, 0);
+    else return p.at(v, dx, 0, 0);
+}
 
 // The plane loop shared by the tile shapes, planes k0 .. k1-1.  load(k, slot,
 // want) fills a ring slot with plane k (want false: an empty resource, the

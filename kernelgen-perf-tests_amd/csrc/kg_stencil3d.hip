@@ -292,7 +292,7 @@ struct GradientOp {
         double rx[C::VEC], ry[C::VEC], rz[C::VEC];
 #pragma unroll
         for (int v = 0; v < C::VEC; v++) {
-            rx[v] = alpha * (p.at(v, 1, 0, 0) - p.at(v, -1, 0, 0));
+            rx[v] = alpha * (kgp::xrow<C>(p, v, 1) - kgp::xrow<C>(p, v, -1));
             ry[v] = beta * (p.at(v, 0, 1, 0) - p.at(v, 0, -1, 0));
             rz[v] = gamma * (p.col(v, 1) - p.col(v, -1));
         }
@@ -359,7 +359,17 @@ struct LapgsrbOp {
 #ifndef KG_LAP_KC
 #define KG_LAP_KC 8
 #endif
-template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0, KG_LAP_KC>;
+#ifndef KG_LAP_XS
+#define KG_LAP_XS 0
+#endif
+#ifndef KG_GRAD_XS
+#define KG_GRAD_XS 0
+#endif
+#ifndef KG_WAVE_XS
+#define KG_WAVE_XS 0
+#endif
+template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0, KG_LAP_KC,
+                                         kgp::F64, 64, false, V == 2 && KG_LAP_XS>;
 // gradient: W=12/D=2/KC=8 in chunks of 2 tile rows, one workgroup (12 waves)
 // per CU: 196 us (W=8 with 2 workgroups 201, 3 workgroups 208; W=12 with 2
 // 213; W=16 with 1 214; KC=16 224, D=3 209, XCD-contiguous 219).
@@ -369,8 +379,10 @@ template <int V> using LapCfg = kgp::Cfg<V, KG_LAP_W, 1, 0, 0, 1, 1, KG_LAP_D, 0
 #ifndef KG_GRAD_OCC
 #define KG_GRAD_OCC 1
 #endif
-template <int V> using GradCfg = kgp::Cfg<V, KG_GRAD_W, 1, 0, 0, 1, 1, 2, 0, 8>;
-template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
+template <int V> using GradCfg = kgp::Cfg<V, KG_GRAD_W, 1, 0, 0, 1, 1, 2, 0, 8, kgp::F64, 64, false,
+                                          V == 2 && KG_GRAD_XS>;
+template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32, kgp::F64, 64, false,
+                                          V == 2 && KG_WAVE_XS>;
 // lapgsrb: 16-byte lanes (even nx, 16-byte aligned fields) take every
 // same-row x-neighbour, in planes k-1 .. k+1, from the neighbouring lanes'
 // registers by DPP shifts (x-shift mode), which leaves LDS only the 12
@@ -381,7 +393,13 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 // 129-137, 4 workgroups per CU 131-139); the same run timed the previous
 // shape (8-byte lanes, every neighbour from LDS, W=12/D=2) at 136 us and
 // 8-byte lanes in the x-shift mode at 149-178 us (twice the DPP moves per
-// point), so odd or unaligned grids keep the previous shape.
+// point), so odd or unaligned grids keep the previous shape.  Back-to-back
+// launches of this kernel slow down after about a dozen (101-107 us, then
+// 125-165 us, with the clock held down under the sustained load; laplacian
+// stays at 98-100 us), so shapes are compared on their first ten launches
+// with cold caches: chunks for one workgroup per CU (a single round of 252
+// workgroups, each with 16 waves) 101-102 us, two per CU 106-107 us; tile
+// orders in chunks of 2 or 4 tile rows alike, per-XCD regions 145-177 us.
 #ifndef KG_GSRB_W
 #define KG_GSRB_W 16
 #endif
@@ -392,7 +410,7 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32>;
 #define KG_GSRB_KC 32
 #endif
 #ifndef KG_GSRB_WGPC
-#define KG_GSRB_WGPC 2
+#define KG_GSRB_WGPC 1
 #endif
 template <int V> using GsrbCfg = kgp::Cfg<V, (V == 2 ? KG_GSRB_W : 12), 2, 1, 1, 2, 2, KG_GSRB_D, 0,
                                           KG_GSRB_KC, kgp::F64, 64, false, V == 2>;

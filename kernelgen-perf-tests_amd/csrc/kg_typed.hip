@@ -21,6 +21,12 @@
 // laplacian 223 us XCD-contiguous, 210 in chunks of 2 tile rows (213-211 in
 // hardware order or chunks of 4, 8), FP32 wave13pt 337 -> 315 us, BF16 within
 // 3 %; per-plane gaussblur 242 us (FP32) in hardware order, 251-280 otherwise.
+// x-shift mode for the FP32/BF16 plane-engine kernels: a lane's 4 (FP32)
+// or 8 (BF16) points take their same-row x-neighbours from its own
+// registers and, for the end points, from the neighbouring lanes by DPP.
+#ifndef KG_TYPED_XS
+#define KG_TYPED_XS 0
+#endif
 #ifndef KG_ORD_TYPED
 #define KG_ORD_TYPED 2
 #endif
@@ -47,7 +53,7 @@ static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
     constexpr int ZLO = Bounds<OpT>::ZLO, ZHI = Bounds<OpT>::ZHI;
     auto go = [&](auto vtag) -> int {
         constexpr int V = decltype(vtag)::value;
-        using C = OpCfg<OpT, T, V, W, D, KC>;
+        using C = OpCfg<OpT, T, V, W, D, KC, 64, (V > 1) && KG_TYPED_XS>;
         using Op = OpT<C>;
         Op op{};
         fill(op);

@@ -313,15 +313,52 @@ def run_poisson(kg, ctx, n: int = 8192, tol: float = 1e-10, seed: int = 11,
     return rep
 
 
+def run_typed(kg, ctx, n: int = 512, steps: int = 10, seed: int = 3) -> dict:
+    """The precision variants alone at n^3: each of laplacian and wave13pt in
+    FP32 and BF16 and per-plane gaussblur in FP64 and FP32 for `steps`
+    launches (levels rotated, cold caches); mean launch time, grid points/s
+    and algorithmic HBM bytes/s per kernel (used for profiling)."""
+    N = n * n * n
+    out = {"grid": [n, n, n], "steps": steps}
+    cases = [("laplacian_f32", np.float32, 2, LAP, 8), ("laplacian_bf16", np.uint16, 2, LAP, 4),
+             ("wave13pt_f32", np.float32, 3, WAVE, 12), ("wave13pt_bf16", np.uint16, 3, WAVE, 6),
+             ("gaussblur3_f64", np.float64, 2, GAUSS, 16), ("gaussblur3_f32", np.float32, 2, GAUSS, 8)]
+    for fn, dt, nlev, co, bpp in cases:
+        code = {np.float64: kg.F64, np.float32: kg.F32, np.uint16: kg.BF16}[dt]
+        lv = [ctx.empty((N,), dt) for _ in range(nlev)]
+        for l, b in enumerate(lv):
+            if code == kg.BF16:
+                tmp = ctx.empty((N,), np.float32)
+                kg._check(ctx.lib, ctx.lib.kg_fill_uniform(ctx.ptr, kg.F32, N, tmp.ptr, seed + l), "fill")
+                kg._check(ctx.lib, ctx.lib.kg_convert(ctx.ptr, kg.BF16, b.ptr, kg.F32, tmp.ptr, N),
+                          "convert")
+                ctx.sync()
+                tmp.free()
+            else:
+                kg._check(ctx.lib, ctx.lib.kg_fill_uniform(ctx.ptr, code, N, b.ptr, seed + l), "fill")
+        ctx.call(fn, n, n, n, *co, *lv)
+        ctx.sync()
+        ctx.drain()
+        for _ in range(steps):
+            ctx.call(fn, n, n, n, *co, *lv)
+            lv = lv[1:] + lv[:1]
+        t = float(np.mean(ctx.drain()))
+        for b in lv:
+            b.free()
+        out[fn] = {"us": round(t * 1e6, 2), "gpts_s": round(N / t / 1e9, 1),
+                   "gbs": round(N * bpp / t / 1e9, 1), "frac_hbm": round(N * bpp / t / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def _main(argv=None) -> int:
-    """python3 kglayers.py {batch,serve,precision,poisson} [--dtype f32|f64]:
+    """python3 kglayers.py {batch,serve,precision,poisson,typed} [--dtype f32|f64]:
     run one layer on device 0 and print its JSON (used for profiling)."""
     import argparse
     import importlib.util
     import json
     from pathlib import Path
     ap = argparse.ArgumentParser()
-    ap.add_argument("layer", choices=["batch", "serve", "precision", "poisson"])
+    ap.add_argument("layer", choices=["batch", "serve", "precision", "poisson", "typed"])
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--save-shapes", help="batch: write the tuned tile shapes to this JSON file")
@@ -346,6 +383,8 @@ def _main(argv=None) -> int:
             r = run_serve(kg, ctx)
         elif args.layer == "precision":
             r = run_precision(kg, ctx)
+        elif args.layer == "typed":
+            r = run_typed(kg, ctx, steps=args.steps)
         else:
             r = run_poisson(kg, ctx)
     print(json.dumps(r), flush=True)
