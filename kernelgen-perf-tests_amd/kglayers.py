@@ -364,6 +364,7 @@ def _main(argv=None) -> int:
     ap.add_argument("--save-shapes", help="batch: write the tuned tile shapes to this JSON file")
     ap.add_argument("--load-shapes", help="batch: apply these tile shapes instead of autotuning")
     ap.add_argument("--no-check", action="store_true", help="batch: skip the sampled result check")
+    ap.add_argument("--tune-reps", type=int, default=2, help="batch: launches per autotuner trial")
     ap.add_argument("--seq-steps", type=int, default=3,
                     help="batch: steps of the one-launch-per-grid comparison (0: none)")
     args = ap.parse_args(argv)
@@ -374,7 +375,8 @@ def _main(argv=None) -> int:
         if args.layer == "batch":
             shapes = json.load(open(args.load_shapes)) if args.load_shapes else None
             r = run_batch(kg, ctx, kg.F64 if args.dtype == "f64" else kg.F32, steps=args.steps,
-                          check=not args.no_check, shapes=shapes, seq_steps=args.seq_steps)
+                          check=not args.no_check, shapes=shapes, seq_steps=args.seq_steps,
+                          tune_reps=args.tune_reps)
             if args.save_shapes:
                 with open(args.save_shapes, "w") as fh:
                     json.dump(r["shapes"], fh)
