@@ -13,10 +13,12 @@
 // vecadd and sincos are one-read-per-input streams: each thread takes one
 // 16-byte piece of every array (two doubles or four floats), workgroups in
 // address order.  matvec keeps the reference's sequential sum per row
-// (bit-identical to an IEEE build): a workgroup owns 128 rows, stages
-// column chunks of A in LDS with coalesced loads, and each lane sums its own
-// row from LDS in column order.
-#include "kg_internal.h"
+// (bit-identical to an IEEE build), one lane per row: for 16-byte aligned
+// rows the lane streams its own row (matvec_rows_k: 512 x 65536 FP64 in
+// 63 us against 251 us for the LDS-staged form); otherwise a workgroup owns
+// 128 rows, stages column chunks of A in LDS with coalesced loads, and each
+// lane sums its own row from LDS in column order.
+#include "kg_plane.h"
 
 #include <algorithm>
 
@@ -121,6 +123,72 @@ void matvec_k(int nx, int ny, const T* __restrict__ A, const T* __restrict__ x, 
     if (j < ny) y[j] = acc;
 }
 
+// Row-streaming form for rows of a multiple of per16 values on 16-byte
+// aligned A (every row starts on a 16-byte boundary): lane j sums row j
+// itself, reading it with 16-byte loads CH values at a time, the next chunk
+// in flight while the current one is summed (two register buffers, named at
+// compile time so no copy waits for a load).  x[i] is the same for every
+// lane and comes through the scalar cache.  One lane per row leaves few
+// waves (65536 rows: 4 per compute unit), so the rows' bytes in flight, not
+// the lane count, decide the speed.  The sum is the reference's: i
+// ascending, product then add.
+template <class T> constexpr int mv_chunk() { return sizeof(T) == 8 ? 32 : 64; }
+
+template <class T>
+__device__ __forceinline__ void mv_load(__amdgpu_buffer_rsrc_t r, unsigned off, T (&buf)[mv_chunk<T>()])
+{
+    constexpr int P = per16<T>();
+#pragma unroll
+    for (int q = 0; q < mv_chunk<T>() / P; q++) {
+        const auto raw = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + 16u * q), 0, 0);
+        const v16<T> v = __builtin_bit_cast(v16<T>, raw);
+#pragma unroll
+        for (int e = 0; e < P; e++) buf[q * P + e] = v[e];
+    }
+}
+
+// FULL: the whole chunk lies in the row (no per-value test, so the uniform
+// x loads merge into wide scalar loads).
+template <class T, bool FULL>
+__device__ __forceinline__ T mv_sum(T acc, const T (&buf)[mv_chunk<T>()], const T* __restrict__ x,
+                                    int i0, int nx)
+{
+#pragma unroll
+    for (int c = 0; c < mv_chunk<T>(); c++)
+        if (FULL || i0 + c < nx) acc = acc + buf[c] * x[i0 + c];
+    return acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(64)
+void matvec_rows_k(int nx, int ny, const T* __restrict__ A, const T* __restrict__ x, T* __restrict__ y,
+                   unsigned a_bytes)
+{
+    constexpr int CH = mv_chunk<T>();
+    const int j = (int)(blockIdx.x * 64 + threadIdx.x);
+    const __amdgpu_buffer_rsrc_t r = kgp::make_rsrc(A, a_bytes);
+    // lanes past the last row read nothing (offset beyond the resource)
+    const unsigned row = j < ny ? (unsigned)((long long)j * nx * (long long)sizeof(T)) : kgp::KG_OOB;
+    T b0[CH], b1[CH];
+    T acc = 0;
+    mv_load<T>(r, row, b0);
+    int i0 = 0;
+    for (; i0 + 2 * CH <= nx; i0 += 2 * CH) {
+        mv_load<T>(r, row + (unsigned)((i0 + CH) * sizeof(T)), b1);
+        acc = mv_sum<T, true>(acc, b0, x, i0, nx);
+        if (i0 + 2 * CH < nx) mv_load<T>(r, row + (unsigned)((i0 + 2 * CH) * sizeof(T)), b0);
+        acc = mv_sum<T, true>(acc, b1, x, i0 + CH, nx);
+    }
+    if (i0 < nx) {   // a last chunk, in b0 (whole or in part)
+        acc = mv_sum<T, false>(acc, b0, x, i0, nx);
+        if (i0 + CH < nx) {
+            mv_load<T>(r, row + (unsigned)((i0 + CH) * sizeof(T)), b1);
+            acc = mv_sum<T, false>(acc, b1, x, i0 + CH, nx);
+        }
+    }
+    if (j < ny) y[j] = acc;
+}
+
 unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 // A launch holds fewer than 2^32 threads: one per 16-byte piece (aligned
@@ -167,9 +235,14 @@ int matvec(kg_ctx* ctx, const char* who, const char* name, int nx, int ny, const
     if (int rc = kg_check_dims2(nx, ny)) return rc;
     if (!A || !x || !y) return kg_set_error(KG_ERR_INVALID, "%s: null buffer", who);
     if (y == A || y == x) return kg_set_error(KG_ERR_INVALID, "%s: y aliases an input", who);
-    kg_register_kernel(name, (const void*)matvec_k<T>);
+    const long long a_bytes = (long long)nx * ny * (long long)sizeof(T);
+    const bool rows16 = nx % per16<T>() == 0 && !((uintptr_t)A & 15) && a_bytes < (1LL << 31);
+    kg_register_kernel(name, rows16 ? (const void*)matvec_rows_k<T> : (const void*)matvec_k<T>);
     if (int rc = kg_launch_begin(ctx, name)) return rc;
-    if (ny > 0)
+    if (ny > 0 && rows16 && nx > 0)
+        hipLaunchKernelGGL(matvec_rows_k<T>, dim3((ny + 63) / 64), dim3(64), 0, ctx->stream, nx, ny,
+                           A, x, y, (unsigned)a_bytes);
+    else if (ny > 0)
         hipLaunchKernelGGL(matvec_k<T>, dim3((ny + MV_ROWS - 1) / MV_ROWS), dim3(MV_ROWS), 0,
                            ctx->stream, nx, ny, A, x, y);
     return kg_launch_end(ctx);
