@@ -315,7 +315,9 @@ int kg_batch_info(kg_batch* b, double* points_per_step, double* bytes_per_step,
  * whole rows packed back to back across the workgroup (lanes_per_row -1, for
  * rows narrower than a wave), the direct row body of the planar kinds
  * (gaussblur3, jacobi2, life2; lanes_per_row -2: rows loaded through the
- * caches, one plane per tile), or
+ * caches, one plane per tile), the column march of the planar kinds
+ * (lanes_per_row -3: each wave marches one row piece of one plane along y in
+ * registers, planes_per_tile then counting the rows per wave), or
  * the flat body (lanes_per_row 0: points swept from cache, for small grids)
  * -- with 8/16/32/64 planes per tile, over reps launches on the batch's own
  * grids, and keeps the fastest.  Untuned classes take the direct row body

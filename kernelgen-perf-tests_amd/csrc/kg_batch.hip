@@ -60,10 +60,11 @@ constexpr int NLPR = 3;         // plane-engine shapes: lanes per row 64, 32, 16
 constexpr int FLAT = NLPR;      // shape index of the flat (cache-served) body
 constexpr int PACK = NLPR + 1;  // shape index of packed whole rows
 constexpr int ROW2 = NLPR + 2;  // shape index of the direct row body (planar kinds only)
-constexpr int NSHAPE = NLPR + 3;
+constexpr int COL = NLPR + 3;   // shape index of the column march (planar kinds only)
+constexpr int NSHAPE = NLPR + 4;
 constexpr int NVAR = NKIND * NTYPE * NSHAPE;
-// API codes: 0 flat, -1 packed rows, -2 direct rows
-constexpr int LPRS[NSHAPE] = {64, 32, 16, 0, -1, -2};
+// API codes: 0 flat, -1 packed rows, -2 direct rows, -3 column march
+constexpr int LPRS[NSHAPE] = {64, 32, 16, 0, -1, -2, -3};
 // Kinds whose stencil reads only the plane it writes (gaussblur3 and the 2D
 // kinds, applied to every xy-plane): they can run the direct row body.
 constexpr bool is_planar(int kind)
@@ -145,6 +146,24 @@ using VarCfg = kgp::Cfg<vec_of(K, DT), BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind
 
 constexpr int var_id(int kind, int dt, int li) { return (kind * NTYPE + dt) * NSHAPE + li; }
 
+// Column march (kgp::colmarch_tile, planar kinds): the 16 waves of a tile
+// form COLZ columns of planes by 16/COLZ chunks of kc rows; each wave keeps
+// rows j-H .. j+H plus COLD prefetched rows of its columns in registers.
+#ifndef KG_BATCH_COLZ
+#define KG_BATCH_COLZ 4
+#endif
+#ifndef KG_BATCH_COLD64
+#define KG_BATCH_COLD64 2
+#endif
+#ifndef KG_BATCH_COLD32
+#define KG_BATCH_COLD32 4
+#endif
+constexpr int COLZ = KG_BATCH_COLZ;
+template <int K, int DT>
+using ColCfg = kgp::Cfg<TypeOf<DT>::VEC, BW, Kind<K>::Sh::H, 0, 0, Kind<K>::Sh::H, Kind<K>::Sh::H,
+                        DT == KG_F64 ? KG_BATCH_COLD64 : KG_BATCH_COLD32, 0, 32,
+                        typename TypeOf<DT>::T, 64, true, true>;
+
 // Flat body configuration: one point per lane, operands from global memory.
 template <int K, int DT>
 using FlatCfg = kgp::Cfg<1, BW, Kind<K>::Sh::H, Kind<K>::Sh::LL, Kind<K>::Sh::LR, Kind<K>::Sh::ZL,
@@ -175,6 +194,14 @@ constexpr VarInfo var_info()
         } else {
             return var_info<K, DT, 0>();   // never used: build_work maps it to shape 0
         }
+    } else if constexpr (LI == COL) {
+        if constexpr (is_planar(K)) {
+            using C = ColCfg<K, DT>;
+            return VarInfo{kgp::ColMarch<C>::TXW, 0, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC,
+                           (int)sizeof(typename C::S), C::H, 0};
+        } else {
+            return var_info<K, DT, 0>();   // never used: build_work maps it to shape 0
+        }
     } else {
         using C = VarCfg<K, DT, LPRS[LI]>;
         return VarInfo{C::TXW, C::TY, B::LO, B::HI, B::ZLO, B::ZHI, C::VEC,
@@ -193,7 +220,8 @@ constexpr VarInfo var_info()
     X(0, 0) X(0, 1) X(1, 0) X(1, 1) X(2, 0) X(2, 1) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
 
 #define KG_VAR_INFO6(K, DT) var_info<K, DT, 0>(), var_info<K, DT, 1>(), var_info<K, DT, 2>(), \
-                            var_info<K, DT, FLAT>(), var_info<K, DT, PACK>(), var_info<K, DT, ROW2>(),
+                            var_info<K, DT, FLAT>(), var_info<K, DT, PACK>(), var_info<K, DT, ROW2>(), \
+                            var_info<K, DT, COL>(),
 constexpr VarInfo VARS[NVAR] = {KG_FLAT_LIST(KG_VAR_INFO6)};
 #define KG_ROW2_LIST(X) X(2, 0) X(2, 1) X(3, 0) X(3, 1) X(4, 0) X(4, 1)
 
@@ -228,6 +256,7 @@ struct GridDesc {
 
 struct Item {
     int grid, bx, by, bz;
+    int var;              // the grid's variant id (read with the item)
 };
 
 struct BatchState {
@@ -363,6 +392,22 @@ __device__ __attribute__((noinline)) void run_rows2(const GridDesc* __restrict__
                                          B::ZLO + bz, op);
 }
 
+template <int K, int DT>
+__device__ __attribute__((noinline)) void run_col(const GridDesc* __restrict__ gp_, int bx_,
+                                                  int by_, int bk_, unsigned long long step_)
+{
+    static_assert(is_planar(K), "column march: kinds that read only their own plane");
+    const GridDesc& g = *uni(gp_);
+    const int bx = uni(bx_), by = uni(by_), bk = uni(bk_);
+    const unsigned long long step = uni(step_);
+    using C = ColCfg<K, DT>;
+    using Op = typename Kind<K>::template Op<C>;
+    using B = typename Kind<K>::B;
+    const Op op = make_op<K, C>(g, step);
+    kgp::colmarch_tile<C, B::LO, B::HI, Op, COLZ>(uni(g.nx), uni(g.ny), uni(g.ns), uni(g.sy),
+                                                  uni(g.sz), bx, by, bk, uni(g.kc), op);
+}
+
 // Flat body: the workgroup sweeps the interior points of planes
 // ZLO + bz*kc .. +kc, one point per lane per round, x fastest; operands come
 // from global memory, where a small grid's planes stay cached (L1/L2).
@@ -413,6 +458,14 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
     // and takes items from part b % 8 first: neighbouring tiles, which share
     // halo rows and planes, then run on one XCD and share its L2.  A
     // workgroup whose part is empty moves on to the next part.
+    // Taking runs of several items per atomic (fewer atomics, neighbouring
+    // tiles in sequence on one workgroup) measured slower: 2, 4, 8 and 32
+    // items per take cost 4 %, 7 %, 10 % and 17 % (FP32).  So did taking the
+    // next item while the current one runs (5 %: the tile's first wait for
+    // its loads then also waits for the atomic) and running a workgroup's
+    // consecutive items of one variant in a loop inside the tile function
+    // (3 %), which saves the callee-saved registers once per run of items
+    // instead of once per item.
     const int home = (int)(blockIdx.x % KG_NUM_XCD);
     int part = home, lo = 0, hi = 0, parts_left = KG_NUM_XCD;
     lo = (int)((long long)part * nitems / KG_NUM_XCD);
@@ -431,7 +484,7 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
         }
         const Item it = items[i];
         const GridDesc* g = grids + it.grid;
-        switch (g->var) {
+        switch (it.var) {
 #define KG_VAR_CASE(K, DT, LI) \
     case var_id(K, DT, LI): run_tile<K, DT, LPRS[LI]>(g, it.bx, it.by, it.bz, step); break;
             KG_VAR_LIST(KG_VAR_CASE)
@@ -442,7 +495,8 @@ void batch_k(const GridDesc* __restrict__ grids, const Item* __restrict__ items,
             KG_FLAT_LIST(KG_FLAT_CASE)
 #undef KG_FLAT_CASE
 #define KG_ROW2_CASE(K, DT) \
-    case var_id(K, DT, ROW2): run_rows2<K, DT>(g, it.bx, it.by, it.bz, step); break;
+    case var_id(K, DT, ROW2): run_rows2<K, DT>(g, it.bx, it.by, it.bz, step); break; \
+    case var_id(K, DT, COL): run_col<K, DT>(g, it.bx, it.by, it.bz, step); break;
             KG_ROW2_LIST(KG_ROW2_CASE)
 #undef KG_ROW2_CASE
         default: break;
@@ -488,6 +542,15 @@ void seq_rows2_k(const GridDesc* __restrict__ gp, int ntx, int nty, unsigned lon
     const unsigned b = blockIdx.x;
     run_rows2<K, DT>(gp, (int)(b % (unsigned)ntx), (int)((b / (unsigned)ntx) % (unsigned)nty),
                      (int)(b / ((unsigned)ntx * (unsigned)nty)), step);
+}
+
+template <int K, int DT>
+__global__ __launch_bounds__(64 * BW, KG_BATCH_WPE)
+void seq_col_k(const GridDesc* __restrict__ gp, int ntx, int nty, unsigned long long step)
+{
+    const unsigned b = blockIdx.x;
+    run_col<K, DT>(gp, (int)(b % (unsigned)ntx), (int)((b / (unsigned)ntx) % (unsigned)nty),
+                   (int)(b / ((unsigned)ntx * (unsigned)nty)), step);
 }
 
 template <int K, int DT>
@@ -695,25 +758,35 @@ void drop_graphs(kg_batch* b)
 // Tiles of grid g with shape (li, kc), appended to out.
 void cut_tiles(const HostGrid& g, int gi, int li, int kc, std::vector<Item>& out)
 {
-    const VarInfo& v = VARS[var_id(g.spec.kind, g.spec.dtype, li)];
+    const int var = var_id(g.spec.kind, g.spec.dtype, li);
+    const VarInfo& v = VARS[var];
     const int nx = g.spec.nx, ny = g.spec.ny, ns = g.spec.ns;
     const int nxi = nx - v.lo - v.hi, nyi = ny - v.lo - v.hi, nzi = ns - v.zlo - v.zhi;
     if (nxi <= 0 || nyi <= 0 || nzi <= 0) return;
     if (li == FLAT) {
-        for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++) out.push_back(Item{gi, 0, 0, bz});
+        for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++) out.push_back(Item{gi, 0, 0, bz, var});
         return;
     }
     if (li == ROW2) {   // one plane per item
         const int ntx = (int)((g.sy + v.txw - 1) / v.txw), nty = (nyi + v.ty - 1) / v.ty;
         for (int bz = 0; bz < nzi; bz++)
             for (int by = 0; by < nty; by++)
-                for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bz});
+                for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bz, var});
+        return;
+    }
+    if (li == COL) {   // COLZ planes by 16/COLZ row chunks of kc rows per item
+        const int ntx = (int)((g.sy + v.txw - 1) / v.txw);
+        const int rows = (BW / COLZ) * kc;
+        const int nty = (nyi + rows - 1) / rows, nkb = (ns + COLZ - 1) / COLZ;
+        for (int bk = 0; bk < nkb; bk++)
+            for (int by = 0; by < nty; by++)
+                for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bk, var});
         return;
     }
     if (li == PACK) {
         const int ty = pack_rows_of(v, g.sy) - 2 * v.h;   // > 0: checked by build_work
         for (int bz = 0; bz < (nzi + kc - 1) / kc; bz++)
-            for (int by = 0; by < (nyi + ty - 1) / ty; by++) out.push_back(Item{gi, 0, by, bz});
+            for (int by = 0; by < (nyi + ty - 1) / ty; by++) out.push_back(Item{gi, 0, by, bz, var});
         return;
     }
     const int ntx = (nx + v.txw - 1) / v.txw;
@@ -721,7 +794,7 @@ void cut_tiles(const HostGrid& g, int gi, int li, int kc, std::vector<Item>& out
     const int nkc = (nzi + kc - 1) / kc;
     for (int bz = 0; bz < nkc; bz++)
         for (int by = 0; by < nty; by++)
-            for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bz});
+            for (int bx = 0; bx < ntx; bx++) out.push_back(Item{gi, bx, by, bz, var});
 }
 
 GridDesc make_desc(const HostGrid& g, int li, int kc)
@@ -806,7 +879,7 @@ int build_work(kg_batch* b, int only_class = -1, int li_force = 0, int kc_force 
         // grids whose rows do not pack take the 64-lane rows instead; the
         // direct row body is for the planar kinds only
         if (li == PACK && !pack_rows_of(VARS[var_id(g.spec.kind, g.spec.dtype, PACK)], g.sy)) li = 0;
-        if (li == ROW2 && !is_planar(g.spec.kind)) li = 0;
+        if ((li == ROW2 || li == COL) && !is_planar(g.spec.kind)) li = 0;
         g.li = li;
         g.kc = kc;
         descs[gi] = make_desc(g, li, kc);
@@ -1242,6 +1315,23 @@ extern "C" int kg_batch_step_sequential(kg_batch* b, int nsteps)
         break;
                     KG_ROW2_LIST(KG_SEQ_ROW2)
 #undef KG_SEQ_ROW2
+                default: break;
+                }
+                continue;
+            }
+            if (g.li == COL) {
+                const GridDesc* gp = b->d_desc + gi;
+                const int rows = (BW / COLZ) * g.kc;
+                const int ntx = (int)((g.sy + v.txw - 1) / v.txw), nty = (nyi + rows - 1) / rows;
+                const int nkb = (g.spec.ns + COLZ - 1) / COLZ;
+                switch (g.spec.kind * NTYPE + g.spec.dtype) {
+#define KG_SEQ_COL(K, DT)                                                                       \
+    case K * NTYPE + DT:                                                                        \
+        hipLaunchKernelGGL((seq_col_k<K, DT>), dim3((unsigned)(ntx * nty * nkb)), dim3(64 * BW), \
+                           0, b->ctx->stream, gp, ntx, nty, step);                              \
+        break;
+                    KG_ROW2_LIST(KG_SEQ_COL)
+#undef KG_SEQ_COL
                 default: break;
                 }
                 continue;

@@ -33,6 +33,11 @@
 // laplacian and 1.22x for lapgsrb: tiles 512 bytes wide re-fetch the x-halo
 // lines their neighbours on other XCDs also read, and the y-halo rows of a
 // tile are read again by the tiles above and below it.
+//
+// The header also holds the engines for stencils that read only their own
+// xy-plane: packed rows (narrow grids), the direct row body (rows loaded
+// through the caches) and the column march (a wave marches a row piece
+// along y in registers, x-neighbours by DPP lane shifts, no LDS).
 #pragma once
 
 #include "kg_internal.h"
@@ -193,7 +198,6 @@ struct Cfg {
     static constexpr size_t LDS_BYTES = sizeof(F) * NLS * ROWS * VEC * LQ;
     static_assert(LPR == 64 || LPR == 32 || LPR == 16, "lanes per row");
     static_assert(LPR > 2 * H, "halo lanes of a row must not overlap");
-    static_assert(TY > 0, "tile has no computed rows");
     static_assert(!XS || (LPR == 64 && H <= 2 * VEC), "x-shift mode: one row per wave, H <= 2 VEC");
 };
 
@@ -274,6 +278,25 @@ __host__ __device__ constexpr int floor_div(int e, int d) { return (e + 8 * d) /
 // register ring's logical slot r (plane k - ZL + r) is physical slot
 // (P + r) % NP, and the side-stream ring's logical slot d (plane k + d) is
 // physical slot (P + d) % NP.
+// Point x0 + e of a row held in a ring slot in the x-shift mode (x0 the
+// piece's first column, e = v + dx, |dx| <= H): component e mod VEC of lane
+// floor(e / VEC); a value m lanes away takes m wave-wide shifts, the lane at
+// the wave's end receiving the halo point the shift brings in (left halo
+// h[i] = x0 - H + i on lane 0, right halo h[i] = x0 + TXW + i on lane 63).
+// All 64 lanes must be active.
+template <class C, class F, int VEC, int NH>
+__device__ __forceinline__ F slot_xat(const Slot<F, VEC, NH>& s, int v, int dx)
+{
+    const int e = v + dx;
+    const int q = floor_div(e, VEC), r = e - q * VEC;
+    F t = s.v[r];
+#pragma unroll
+    for (int m = 1; m <= -q; m++) t = dpp_shift<KG_DPP_WAVE_SHR1>(s.h[C::H - m * VEC + r], t);
+#pragma unroll
+    for (int m = 1; m <= q; m++) t = dpp_shift<KG_DPP_WAVE_SHL1>(s.h[(m - 1) * VEC + r], t);
+    return t;
+}
+
 template <class C, int P = 0>
 struct View {
     using F = typename C::F;
@@ -308,15 +331,7 @@ struct View {
     __device__ __forceinline__ F xat(int v, int dx, int dz) const
     {
         static_assert(C::XS, "xat needs the x-shift mode");
-        const Slot<F, C::VEC, C::NH>& s = ring[(P + C::ZLO + dz) % C::NP];
-        const int e = v + dx;
-        const int q = floor_div(e, C::VEC), r = e - q * C::VEC;
-        F t = s.v[r];
-#pragma unroll
-        for (int m = 1; m <= -q; m++) t = dpp_shift<KG_DPP_WAVE_SHR1>(s.h[C::H - m * C::VEC + r], t);
-#pragma unroll
-        for (int m = 1; m <= q; m++) t = dpp_shift<KG_DPP_WAVE_SHL1>(s.h[(m - 1) * C::VEC + r], t);
-        return t;
+        return slot_xat<C>(ring[(P + C::ZLO + dz) % C::NP], v, dx);
     }
     __device__ __forceinline__ F side(int a, int v) const { return aux[P % C::NP].a[a][v]; }
 };
@@ -344,7 +359,7 @@ __device__ __forceinline__ typename C::F xrow(const V& p, int v, int dx)
 // wait for loads issued in the same iteration.  The planes the ring and LDS
 // need are all < ns (the stencil reach ZHI <= the z-boundary width), so
 // slots whose load is skipped are never read.
-template <class C, class Load, class LoadAux, class Stage, class Compute>
+template <class C, bool SYNC = true, class Load, class LoadAux, class Stage, class Compute>
 __device__ __forceinline__ void march(int k0, int k1, Load&& load, LoadAux&& load_aux,
                                       Stage&& stage, Compute&& compute)
 {
@@ -383,7 +398,7 @@ __device__ __forceinline__ void march(int k0, int k1, Load&& load, LoadAux&& loa
         const int kp = k + C::ZHI + C::D;   // plane entering the ring
         load(kp, ring[(P + C::RS) % C::NP], kp <= k1 - 1 + C::ZHI);
         load_aux(k + C::D, aux[(P + C::D) % C::NP], k + C::D < k1);
-        __syncthreads();
+        if constexpr (SYNC) __syncthreads();
         compute(phase, k, ring, aux);
     };
     int k = k0;
@@ -408,6 +423,7 @@ __device__ __forceinline__ void plane_tile(typename C::F* lds, int nx, int ny,
                                            int ns, long long sy, long long sz, int bx, int by,
                                            int bz, int kc, const Op& op)
 {
+    static_assert(C::TY > 0, "tile has no computed rows");
     using S = typename C::S;
     using F = typename C::F;
     using T = typename C::T;
@@ -705,6 +721,105 @@ void rows2_kernel(int nx, int ny, int ntx, int nty, Op op)
     rows2_tile<C, LO, HI, Op, PY>(nx, ny, nx, (long long)nx * ny, bx, by, k, op);
 }
 
+// ------------------------------------------------------------ column march
+// For stencils that read only the xy-plane they write (gaussblur per plane,
+// the 2D kinds).  A wave owns one row piece of 64*VEC points of one plane
+// and marches it along y through a chunk of rows, keeping rows j-H .. j+H
+// of its own columns (plus the D-row prefetch) in a register ring; the
+// x-neighbours come from the adjacent lanes by wave-wide DPP shifts, lane 0
+// holding the H points left of the piece and lane 63 the H points right of
+// it.  Nothing is staged in LDS and there are no barriers, so the waves of a
+// workgroup run independently.  C: x-shift mode, LPR = 64, LL = LR = 0,
+// ZL = ZR = H (the ring's "planes" are rows).
+template <class C, int P = 0>
+struct CView {
+    using F = typename C::F;
+    const Slot<F, C::VEC, C::NH>* ring;   // [NP]
+    bool wr;                              // the lane's points lie in the (padded) row
+    // (x + v + dx, j + dy) of the lane's plane; |dx|, |dy| <= H
+    __device__ __forceinline__ F at(int v, int dx, int dy, int) const
+    {
+        return slot_xat<C>(ring[(P + C::ZLO + dy) % C::NP], v, dx);
+    }
+    __device__ __forceinline__ F col(int v, int) const { return ring[(P + C::ZLO) % C::NP].v[v]; }
+    __device__ __forceinline__ F side(int, int) const { return F(0); }
+    template <class S>
+    __device__ __forceinline__ void put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                        const bool (&xm)[C::VEC]) const;
+};
+
+template <class C>
+struct ColMarch {
+    static_assert(C::XS && C::LPR == 64 && C::LL == 0 && C::LR == 0 && C::ZL == C::H &&
+                  C::ZR == C::H && C::NAUX == 0 && C::H <= C::VEC, "column march configuration");
+    static constexpr int TXW = 64 * C::VEC;   // piece width
+};
+
+// Tile (bx, by, bk) of the xy-interior [LO, n-HI) x all planes: the W
+// waves form WZ columns of planes by W/WZ chunks of rows; wave w marches the
+// kc rows of row chunk by*(W/WZ) + w/WZ in plane bk*WZ + w%WZ.  Every lane
+// of a wave with work stays active to the end (the DPP shifts read all 64
+// lanes); waves without rows return at once.
+template <class C, int LO, int HI, class Op, int WZ = 1>
+__device__ __forceinline__ void colmarch_tile(int nx, int ny, int ns, long long sy, long long sz,
+                                              int bx, int by, int bk, int kc, const Op& op)
+{
+    static_assert(C::W % WZ == 0, "waves form WZ whole columns");
+    using S = typename C::S;
+    using F = typename C::F;
+    using T = typename C::T;
+    (void)ColMarch<C>{};
+    const int lane = (int)(threadIdx.x & 63);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int k = bk * WZ + w % WZ;
+    const int y0 = LO + (by * (C::W / WZ) + w / WZ) * kc;
+    const int y1 = min(y0 + kc, ny - HI);
+    if (k >= ns || y0 >= y1) return;
+    const int x0 = bx * ColMarch<C>::TXW;
+    const int x = x0 + lane * C::VEC;
+    const unsigned off_v = x < nx ? (unsigned)((long long)x * (long long)sizeof(S)) : KG_OOB;
+    // the H halo points of lane 0 (left of the piece) or lane 63 (right of
+    // it) in one load: they lie in one row whenever the first does (nx and
+    // x0 are multiples of VEC >= H), and start H*sizeof(S)-aligned
+    const int hx = lane == 0 ? x0 - C::H : lane == 63 ? x0 + ColMarch<C>::TXW : -1;
+    const unsigned off_h = hx >= 0 && hx < nx ? (unsigned)((long long)hx * (long long)sizeof(S)) : KG_OOB;
+    const S* plane = op.src + (long long)k * sz;
+    const unsigned row_bytes = (unsigned)(sy * (long long)sizeof(S));
+    // row jr of the plane through its own resource; empty unless want and
+    // 0 <= jr < ny
+    auto load = [&](int jr, Slot<F, C::VEC, C::NH>& s, bool want) {
+        const bool ok = want && jr >= 0 && jr < ny;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane + (long long)(ok ? jr : 0) * sy,
+                                                   ok ? row_bytes : 0u);
+        bload<T, C::VEC>(r, off_v, s.v);
+        bload<T, C::NH>(r, off_h, s.h);
+    };
+    auto no_aux = [&](int, ASlot<F, C::VEC, C::NA>&, bool) {};
+    auto no_stage = [&](int, const Slot<F, C::VEC, C::NH>&) {};
+    bool xmask[C::VEC];
+#pragma unroll
+    for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
+    march<C, false>(y0, y1, load, no_aux, no_stage,
+                    [&](auto phase, int j, const auto& ring, const auto&) {
+        constexpr int P = decltype(phase)::value;
+        CView<C, P> vw{ring, x < sy};
+        op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
+    });
+}
+
+// Single-grid launch of the column march over every plane of a dense grid:
+// tiles (bx, by, k), x fastest, then y, then plane, in hardware order.
+template <class C, int LO, int HI, class Op>
+__global__ __launch_bounds__(64 * C::W)
+void colmarch_kernel(int nx, int ny, int ns, int ntx, int nty, int kc, Op op)
+{
+    const unsigned b = blockIdx.x;
+    const int bx = (int)(b % (unsigned)ntx);
+    const int by = (int)((b / (unsigned)ntx) % (unsigned)nty);
+    const int k = (int)(b / ((unsigned)ntx * (unsigned)nty));
+    colmarch_tile<C, LO, HI, Op>(nx, ny, ns, nx, (long long)nx * ny, bx, by, k, kc, op);
+}
+
 // Single-grid launch in the dense layout: one workgroup per tile of kc
 // planes, tiles enumerated x fastest, then y, then z-chunk, in tile order ORD
 // (kg_tile_order: hardware, XCD-contiguous or chunks of ORD tile rows).
@@ -822,6 +937,13 @@ template <class C, int R, int NR>
 template <class S>
 __device__ __forceinline__ void RView<C, R, NR>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
                                                      const bool (&xm)[C::VEC]) const
+{
+    put_row<C>(*this, dst, ob, x, r, xm);
+}
+template <class C, int P>
+template <class S>
+__device__ __forceinline__ void CView<C, P>::put(S* dst, long long ob, int x, F (&r)[C::VEC],
+                                                 const bool (&xm)[C::VEC]) const
 {
     put_row<C>(*this, dst, ob, x, r, xm);
 }

@@ -30,6 +30,24 @@
 #ifndef KG_ORD_TYPED
 #define KG_ORD_TYPED 2
 #endif
+#ifndef KG_GB3_W64
+#define KG_GB3_W64 4
+#endif
+#ifndef KG_GB3_D64
+#define KG_GB3_D64 2
+#endif
+#ifndef KG_GB3_YC64
+#define KG_GB3_YC64 16
+#endif
+#ifndef KG_GB3_W32
+#define KG_GB3_W32 4
+#endif
+#ifndef KG_GB3_D32
+#define KG_GB3_D32 4
+#endif
+#ifndef KG_GB3_YC32
+#define KG_GB3_YC32 16
+#endif
 #ifndef KG_ORD_ROWS2
 #define KG_ORD_ROWS2 0
 #endif
@@ -96,6 +114,42 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
     if (nxi > 0 && nyi > 0 && ns > 0)
         hipLaunchKernelGGL(kern, dim3((unsigned)ntx * (unsigned)nty * (unsigned)ns), dim3(64 * C::W),
                            0, ctx->stream, nx, ny, ntx, nty, op);
+    return kg_launch_end(ctx);
+}
+
+// Per-plane stencils on the column march (kgp::colmarch_tile): W waves per
+// workgroup, each marching YC rows of one plane with a D-row prefetch.  Same
+// requirements as the direct row body (nx % VEC == 0, 16-byte aligned
+// fields), else the plane engine.  gaussblur3 at 512^3, levels rotated,
+// first ten launches: FP32 W=4/D=4/YC=16 214 us against 255-264 on the
+// direct row body (YC=8 214, 12 225, 32 238, 64 256: fewer workgroups leave
+// a tail); FP64 W=4/D=2/YC=16 418 us against 451-472.
+#ifndef KG_GB3_ENGINE
+#define KG_GB3_ENGINE 1
+#endif
+template <template <class> class OpT, class T, int VEC, int W, int D, int YC, class Fill>
+static int launch_colmarch(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
+{
+    if (nx % VEC != 0 || !al) return launch_typed<OpT, T, VEC, 8, 2, 16>(ctx, name, nx, ny, ns, al, fill);
+    if (int rc = kg_check_ctx(ctx)) return rc;
+    if (int rc = kg_check_dims3(nx, ny, ns)) return rc;
+    if (int rc = kg_check_plane_bytes((long long)nx * ny, (int)sizeof(typename T::S))) return rc;
+    constexpr int LO = Bounds<OpT>::LO, HI = Bounds<OpT>::HI;
+    static_assert(Bounds<OpT>::ZLO == 0 && Bounds<OpT>::ZHI == 0, "every plane computed");
+    constexpr int H = Shape<OpT>::H;
+    using C = kgp::Cfg<VEC, W, H, 0, 0, H, H, D, 0, YC, T, 64, false, true>;
+    using Op = OpT<C>;
+    Op op{};
+    fill(op);
+    const int nxi = nx - LO - HI, nyi = ny - LO - HI;
+    const int ntx = (nx + kgp::ColMarch<C>::TXW - 1) / kgp::ColMarch<C>::TXW;
+    const int nty = nyi > 0 ? (nyi + W * YC - 1) / (W * YC) : 0;
+    auto kern = kgp::colmarch_kernel<C, LO, HI, Op>;
+    kg_register_kernel(name, (const void*)kern);
+    if (int rc = kg_launch_begin(ctx, name)) return rc;
+    if (nxi > 0 && nyi > 0 && ns > 0)
+        hipLaunchKernelGGL(kern, dim3((unsigned)ntx * (unsigned)nty * (unsigned)ns), dim3(64 * W),
+                           0, ctx->stream, nx, ny, ns, ntx, nty, YC, op);
     return kg_launch_end(ctx);
 }
 
@@ -367,7 +421,11 @@ extern "C" int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0,
     KG_REQUIRE(w0 && w1, "kg_gaussblur3_f64: null buffer");
     KG_REQUIRE(w0 != w1, "kg_gaussblur3_f64: w0 and w1 must be distinct");
     const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
+#if KG_GB3_ENGINE == 1
+    return launch_colmarch<Gaussblur3Op, kgp::F64, 2, KG_GB3_W64, KG_GB3_D64, KG_GB3_YC64>(ctx, "gaussblur3_f64", nx, ny, ns,
+#else
     return launch_rows2<Gaussblur3Op, kgp::F64, 2, 8, 4>(ctx, "gaussblur3_f64", nx, ny, ns,
+#endif
                                                              kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;
@@ -387,7 +445,11 @@ extern "C" int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, 
     KG_REQUIRE(w0 && w1, "kg_gaussblur3_f32: null buffer");
     KG_REQUIRE(w0 != w1, "kg_gaussblur3_f32: w0 and w1 must be distinct");
     const float f = 1.f / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);
+#if KG_GB3_ENGINE == 1
+    return launch_colmarch<Gaussblur3Op, kgp::F32, 4, KG_GB3_W32, KG_GB3_D32, KG_GB3_YC32>(ctx, "gaussblur3_f32", nx, ny, ns,
+#else
     return launch_rows2<Gaussblur3Op, kgp::F32, 4, 8, 8>(ctx, "gaussblur3_f32", nx, ny, ns,
+#endif
                                                              kg_aligned16(w0, w1), [&](auto& op) {
         op.src = w0;
         op.dst = w1;

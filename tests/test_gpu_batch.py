@@ -181,11 +181,13 @@ def test_batch_shapes_and_tuning_do_not_change_results(gpu_ctx, batch):
     _check_all(gpu_ctx, batch, specs, ids, init, 18, "after tuning")
 
 
-@pytest.mark.parametrize("shape,kc", [(-1, 3), (-1, 16), (-2, 8)])
+@pytest.mark.parametrize("shape,kc", [(-1, 3), (-1, 16), (-2, 8), (-3, 1), (-3, 5), (-3, 16)])
 def test_batch_packed_and_direct_rows_match_single_grid(gpu_ctx, batch, shape, kc):
     """Every grid in packed-row tiles (-1: whole rows back to back across the
-    workgroup) or, for gaussblur3, the direct row body (-2; the other kinds
-    then take 64 lanes per row): rows of 3..69 points, FP64 and FP32."""
+    workgroup) or, for gaussblur3, the direct row body (-2) or the column
+    march (-3: kc rows per wave; the other kinds then take 64 lanes per row):
+    rows of 3..69 points, FP64 and FP32, grids of fewer planes than a tile
+    has plane columns."""
     rng = np.random.default_rng(40 + kc)
     specs = _mixed_specs(rng, 24) + [("wave13pt", F64, (5, 70, 6)), ("gaussblur3", F32, (69, 5, 3)),
                                      ("laplacian", F32, (3, 3, 3)), ("gaussblur3", F64, (300, 41, 2)),
@@ -228,11 +230,11 @@ def test_batch_rejects_bad_specs(kgmod, batch):
         batch.remove(12345)
 
 
-@pytest.mark.parametrize("lpr", [None, -1])
+@pytest.mark.parametrize("lpr", [None, -1, -3])
 def test_batch_fill_and_sequential_path(gpu_ctx, kgmod, lpr):
     """kg_batch_fill is deterministic in the seed; the per-grid sequential
-    launches give the persistent kernel's results (default shapes, and every
-    grid in packed rows)."""
+    launches give the persistent kernel's results (default shapes, every
+    grid in packed rows, every gaussblur3 grid in the column march)."""
     rng = np.random.default_rng(6)
     specs = _mixed_specs(rng, 9, 4, 60)
     outs = []
@@ -254,3 +256,27 @@ def test_batch_fill_and_sequential_path(gpu_ctx, kgmod, lpr):
         assert f0.tobytes() == f1.tobytes()
         assert np.all(np.abs(f0) <= 1) and np.std(f0) > 0.3
         assert o0.tobytes() == o1.tobytes(), (k, d, dims)
+
+
+@pytest.mark.parametrize("dtype", [F64, F32])
+def test_batch_column_march_2d_kinds(gpu_ctx, kgmod, dtype):
+    """jacobi2 and life2 grids (one 2D stencil per plane) give the same bits
+    in the column march (-3) as in the direct row body (-2) and the plane
+    engine (64 lanes per row), after 6 steps; odd widths, 1 to 9 planes."""
+    coef = {"jacobi2": (0.0, 0.2, 0.05), "life2": ()}
+    dims = [(130, 67, 1), (67, 130, 3), (515, 9, 9), (4, 300, 2), (129, 33, 5)]
+    outs = []
+    for shape, kc in ((-2, 8), (-3, 1), (-3, 7), (-3, 32), (64, 8)):
+        b = kgmod.Batch(gpu_ctx)
+        ids = []
+        for kind in ("jacobi2", "life2"):
+            for d in dims:
+                g = b.add(kind, dtype, d, coef[kind])
+                b.set_shape(g, shape, kc)
+                ids.append(g)
+        b.fill(77)
+        b.step(6, graph=True)
+        outs.append([b.download(g).tobytes() for g in ids])
+        b.close()
+    for o in outs[1:]:
+        assert o == outs[0]
