@@ -401,8 +401,10 @@ extern "C" int kg_wave13pt_bf16(kg_ctx* ctx, int nx, int ny, int ns, float m0, f
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt_bf16: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_bf16: w2 aliases an input");
-    // 8-byte lanes: with 16-byte lanes (8 points) the 1024-thread tile needs
-    // more than the 128 registers a lane may hold at that size and spills.
+    // 16-byte lanes (8 points) in 12-wave workgroups (768 threads, so up to
+    // 168 registers a lane: the 16-wave tile would spill at 128), 64 planes
+    // per workgroup, 2-plane prefetch; the measurements are listed with the
+    // KG_WBF_* defaults above.
     return launch_typed<Wave13ptOp, kgp::BF16, KG_WBF_V, KG_WBF_W, KG_WBF_D, KG_WBF_KC>(ctx, "wave13pt_bf16", nx, ny, ns,
                                                              kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
