@@ -115,6 +115,22 @@ def test_typed_full_size_one_step(gpu_ctx, oracle, which, dtype):
         assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
 
 
+@pytest.mark.parametrize("dims", [(264, 150, 3), (520, 71, 2), (128, 200, 1)])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_gaussblur3_column_march_pieces_and_chunks(gpu_ctx, oracle, dtype, dims):
+    """Per-plane gaussblur on the column march: rows of several pieces (the
+    x-neighbours across a piece boundary come from the halo points the end
+    lanes load, and a last piece mostly past the row), row chunks of 16 rows
+    per wave with a partial last workgroup, one to three planes; three steps,
+    bit for bit against the oracle."""
+    fn = f"gaussblur3_{dtype}"
+    ins, out, coefs, rot = _case("gaussblur3", dtype, dims, 31)
+    want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 3, rot, f64 if dtype == "f64" else f32)
+    got = _run_gpu(gpu_ctx, fn, dims, coefs, ins, out, 3, rot)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), (fn, dims)
+
+
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("laplacian", "bf16"),
                                          ("wave13pt", "f32"), ("wave13pt", "bf16")])
 def test_typed_z_chunk_seams(gpu_ctx, oracle, which, dtype):
