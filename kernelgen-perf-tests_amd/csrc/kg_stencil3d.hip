@@ -343,7 +343,8 @@ struct LapgsrbOp {
 // the prefetch ring kept in flight (rotating slots, buffer loads): laplacian in
 // chunks of 2 tile rows per XCD, W=12/D=2/KC=8 100 us (W=8 102, W=16 102,
 // D=3 102, KC=16 105, KC=4 107; XCD-contiguous order W=8 106; two or four
-// rows per wave, 64- and 32-point tiles with fewer y-halo rows, 101-107);
+// rows per wave, 64- and 32-point tiles with fewer y-halo rows, 101-107;
+// z-chunks lengthened to 2-8 workgroups per CU, 14-64 planes, 102-108);
 // wave13pt
 // W=12/D=3 141 us (W=16 148, W=14 169; z-chunks lengthened to 2 workgroups
 // per CU 172); lapgsrb with 8-byte lanes, W=12/D=2, hardware order, z-chunks
