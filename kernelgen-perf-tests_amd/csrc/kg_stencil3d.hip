@@ -403,6 +403,8 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32, kgp::F
 // with cold caches: chunks for one workgroup per CU (a single round of 252
 // workgroups, each with 16 waves) 101-102 us, two per CU 106-107 us; tile
 // orders in chunks of 2 or 4 tile rows alike, per-XCD regions 145-177 us.
+// 32-byte lanes (4 points, two 16-byte accesses, half the DPP moves per
+// point; 12 waves, 155 VGPRs: 16 waves spill) 122-128 us, bit-identical.
 #ifndef KG_GSRB_W
 #define KG_GSRB_W 16
 #endif
