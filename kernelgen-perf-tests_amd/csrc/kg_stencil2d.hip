@@ -141,7 +141,9 @@ struct WhisperingOp {
 // Row shapes from sweeps at 512x65536 (cold caches, 40-launch means) with
 // the prefetch rings kept in flight: jacobi/gameoflife D=2/YC=64 113 us (D=4
 // 141, W=8 118); gaussblur D=2/YC=32 116 us (D=3/YC=64 140); whispering with
-// 16-byte lanes, W=4/D=2/YC=8 589 us (8-byte lanes 610-643).
+// 16-byte lanes, W=4/D=2/YC=8 589 us (8-byte lanes 610-643; later, first ten
+// launches of each: YC=4 584 us, 6 616, 12 644; D=1 633, D=3 584; hardware
+// and chunked tile orders 606 and 586, against 585-588 for this shape).
 //                     VEC W      H  NF LL LR ZL ZR D  NAUX YC
 template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 2, 0, 64>;
 template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 2, 0, 32>;
