@@ -745,6 +745,13 @@ struct kg_batch {
     int nwg = 0;                               // persistent workgroups
     std::map<int, hipGraphExec_t> graphs;      // steps per replay -> graph
     double pts_per_step = 0, bytes_per_step = 0;
+    // pinned staging of the work list's uploads; staged marks the end of the
+    // last upload on the stream (the staging memory is rewritten only after it)
+    GridDesc* h_desc = nullptr;
+    size_t h_desc_cap = 0;
+    Item* h_items = nullptr;
+    size_t h_item_cap = 0;
+    hipEvent_t staged = nullptr;
 };
 
 namespace {
@@ -897,13 +904,37 @@ int build_work(kg_batch* b, int only_class = -1, int li_force = 0, int kc_force 
     if (int rc = ensure_device_arrays(b, (int)std::max<size_t>(descs.size(), 1),
                                       (int)std::max<size_t>(items.size(), 1)))
         return rc;
-    if (!descs.empty())
-        KG_HIP(hipMemcpyAsync(b->d_desc, descs.data(), descs.size() * sizeof(GridDesc),
+    // Uploads go through pinned staging buffers and stay asynchronous: the
+    // work list can change between steps (grids joining and leaving a
+    // serving batch) without draining the stream.  Only the previous upload
+    // must have left the staging memory before it is rewritten.
+    if (b->staged) KG_HIP(hipEventSynchronize(b->staged));
+    else KG_HIP(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+    if (descs.size() > b->h_desc_cap) {
+        if (b->h_desc) KG_HIP(hipHostFree(b->h_desc));
+        b->h_desc = nullptr;
+        const size_t cap = std::max(descs.size(), 2 * b->h_desc_cap);
+        KG_HIP(hipHostMalloc((void**)&b->h_desc, cap * sizeof(GridDesc), hipHostMallocDefault));
+        b->h_desc_cap = cap;
+    }
+    if (items.size() > b->h_item_cap) {
+        if (b->h_items) KG_HIP(hipHostFree(b->h_items));
+        b->h_items = nullptr;
+        const size_t cap = std::max(items.size(), 2 * b->h_item_cap);
+        KG_HIP(hipHostMalloc((void**)&b->h_items, cap * sizeof(Item), hipHostMallocDefault));
+        b->h_item_cap = cap;
+    }
+    if (!descs.empty()) {
+        std::memcpy(b->h_desc, descs.data(), descs.size() * sizeof(GridDesc));
+        KG_HIP(hipMemcpyAsync(b->d_desc, b->h_desc, descs.size() * sizeof(GridDesc),
                               hipMemcpyHostToDevice, b->ctx->stream));
-    if (!items.empty())
-        KG_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(Item),
+    }
+    if (!items.empty()) {
+        std::memcpy(b->h_items, items.data(), items.size() * sizeof(Item));
+        KG_HIP(hipMemcpyAsync(b->d_items, b->h_items, items.size() * sizeof(Item),
                               hipMemcpyHostToDevice, b->ctx->stream));
-    KG_HIP(hipStreamSynchronize(b->ctx->stream));   // host vectors go out of scope
+    }
+    KG_HIP(hipEventRecord(b->staged, b->ctx->stream));
     b->nitems = (int)items.size();
     if (only_class < 0) {
         b->pts_per_step = pts;
@@ -1401,6 +1432,9 @@ extern "C" int kg_batch_destroy(kg_batch* b)
     if (b->d_desc) (void)hipFree(b->d_desc);
     if (b->d_items) (void)hipFree(b->d_items);
     if (b->d_state) (void)hipFree(b->d_state);
+    if (b->h_desc) (void)hipHostFree(b->h_desc);
+    if (b->h_items) (void)hipHostFree(b->h_items);
+    if (b->staged) (void)hipEventDestroy(b->staged);
     delete b;
     return KG_OK;
 }
@@ -1441,6 +1475,8 @@ size_t job_bytes(const kg_job& j)
 struct Active {
     int job, id;
     long long done;
+    long long check_at = -1;       // step of a convergence check in flight
+    long long converged_at = -1;   // step at which the check found convergence
 };
 
 }  // namespace
@@ -1482,11 +1518,19 @@ extern "C" int kg_serve_run(kg_ctx* ctx, const kg_job* jobs, int njobs, size_t h
     int rc = KG_OK;
     const auto t0 = std::chrono::steady_clock::now();
 
-    auto finish = [&](const Active& a) -> int {
+    // A job ends with the field of step `upto` (<= a.done): a jacobi2 job
+    // found converged one step late ends with its previous level, which that
+    // step only read.
+    auto finish = [&](const Active& a, long long upto) -> int {
         const kg_job& j = jobs[a.job];
         kg_job_result& r = results[a.job];
         int lev = 0;
         if (int e = kg_batch_newest_level(b, a.id, &lev)) return e;
+        if (upto != a.done) {
+            if (nlev_of(j.kind) != 2 || upto != a.done - 1)
+                return kg_set_error(KG_ERR_INVALID, "kg_serve_run: no level for step %lld", upto);
+            lev ^= 1;
+        }
         const HostGrid& g = b->grids[a.id];
         double st[5];
         // checksum over the padded level: the padding columns are never
@@ -1500,12 +1544,27 @@ extern "C" int kg_serve_run(kg_ctx* ctx, const kg_job* jobs, int njobs, size_t h
             if (int e = kg_batch_download(b, a.id, lev, j.out)) return e;
         if (int e = kg_batch_remove(b, a.id)) return e;   // waits for the device
         r.finished = now_s(t0);
-        r.steps_run = a.done;
+        r.steps_run = upto;
         in_use -= job_bytes(j);
         finished++;
-        points += (double)j.nx * j.ny * j.ns * (double)a.done;
+        points += (double)j.nx * j.ny * j.ns * (double)upto;
         return KG_OK;
     };
+    // Convergence checks (jacobi2 jobs with tol > 0, every KG_SERVE_CHECK
+    // steps) are issued after a tick and read after the next tick has been
+    // queued: waiting for them then leaves the device busy with that tick.
+    double* check_res = nullptr;   // pinned: 5 statistics per job
+    hipEvent_t checked = nullptr;
+    bool checks_live = false;
+    const double* stats_res = ctx->stats + kg_stats_scratch_bytes() / sizeof(double) - 5;
+    if (hipHostMalloc((void**)&check_res, sizeof(double) * 5 * (size_t)std::max(njobs, 1),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&checked, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (check_res) (void)hipHostFree(check_res);
+        kg_batch_destroy(b);
+        return kg_set_error(KG_ERR_NOMEM, "kg_serve_run: pinned check buffer");
+    }
 
     while (finished < njobs && !rc) {
         const double t = now_s(t0);
@@ -1544,7 +1603,7 @@ extern "C" int kg_serve_run(kg_ctx* ctx, const kg_job* jobs, int njobs, size_t h
         // jobs with nothing to do leave at once
         for (size_t a = 0; a < active.size() && !rc;) {
             if (active[a].done >= jobs[active[a].job].steps) {
-                rc = finish(active[a]);
+                rc = finish(active[a], active[a].done);
                 active.erase(active.begin() + a);
             } else {
                 a++;
@@ -1562,27 +1621,55 @@ extern "C" int kg_serve_run(kg_ctx* ctx, const kg_job* jobs, int njobs, size_t h
         rc = mode == KG_SERVE_SEQUENTIAL ? kg_batch_step_sequential(b, 1) : kg_batch_step(b, 1, 0);
         if (rc) break;
         for (auto& a : active) a.done++;
+        if (checks_live) {
+            if (hipEventSynchronize(checked) != hipSuccess) {
+                rc = kg_set_error(KG_ERR_HIP, "kg_serve_run: convergence check failed");
+                break;
+            }
+            checks_live = false;
+            for (auto& a : active)
+                if (a.check_at >= 0) {
+                    if (check_res[5 * a.job + 4] < jobs[a.job].tol) a.converged_at = a.check_at;
+                    a.check_at = -1;
+                }
+        }
+        bool issued = false;
         for (size_t a = 0; a < active.size() && !rc;) {
-            const kg_job& j = jobs[active[a].job];
-            bool done = active[a].done >= j.steps;
-            if (!done && j.kind == KG_KIND_JACOBI2 && j.tol > 0 && active[a].done % KG_SERVE_CHECK == 0) {
-                const HostGrid& g = b->grids[active[a].id];
-                int lev = 0;
-                if ((rc = kg_batch_newest_level(b, active[a].id, &lev))) break;
-                double st[5];
-                if ((rc = kg_field_stats(ctx, j.dtype, (size_t)g.sz * j.ns, g.lev[lev], j.dtype,
-                                         g.lev[lev ^ 1], st)))
-                    break;
-                done = st[4] < j.tol;
-            }
-            if (done) {
-                rc = finish(active[a]);
+            Active& x = active[a];
+            const kg_job& j = jobs[x.job];
+            if (x.converged_at >= 0 || x.done >= j.steps) {
+                rc = finish(x, x.converged_at >= 0 ? x.converged_at : x.done);
                 active.erase(active.begin() + a);
-            } else {
-                a++;
+                continue;
             }
+            if (j.kind == KG_KIND_JACOBI2 && j.tol > 0 && x.done % KG_SERVE_CHECK == 0) {
+                // largest update of this step: max |w_done - w_(done-1)|
+                const HostGrid& g = b->grids[x.id];
+                int lev = 0;
+                if ((rc = kg_batch_newest_level(b, x.id, &lev))) break;
+                if ((rc = kg_field_stats_launch(ctx->stream, j.dtype, (size_t)g.sz * j.ns, g.lev[lev],
+                                                j.dtype, g.lev[lev ^ 1], ctx->stats)))
+                    break;
+                if (hipMemcpyAsync(check_res + 5 * x.job, stats_res, 5 * sizeof(double),
+                                   hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) {
+                    rc = kg_set_error(KG_ERR_HIP, "kg_serve_run: convergence check copy failed");
+                    break;
+                }
+                x.check_at = x.done;
+                issued = true;
+            }
+            a++;
+        }
+        if (!rc && issued) {
+            if (hipEventRecord(checked, ctx->stream) != hipSuccess)
+                rc = kg_set_error(KG_ERR_HIP, "kg_serve_run: event record failed");
+            else
+                checks_live = true;
         }
     }
+    if (checks_live) (void)hipEventSynchronize(checked);
+    (void)hipEventDestroy(checked);
+    (void)hipHostFree(check_res);
     const double wall = now_s(t0);
     if (rc) {
         kg_batch_destroy(b);

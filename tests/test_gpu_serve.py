@@ -88,6 +88,16 @@ def test_serve_jacobi_stops_at_tolerance(gpu_ctx, kgmod):
                        c2 * (w[:-2, :-2] + w[2:, :-2] + w[:-2, 2:] + w[2:, 2:]))
     assert np.max(np.abs(nxt - w)) < 1e-8
 
+    # the job ends with the field of step n exactly (the check that stops it
+    # is read one step later), and the check at step n - 10 had not converged
+    def fixed(k):
+        r, _, o = kgmod.serve(gpu_ctx, [dict(j, steps=k, tol=0.0)], 1 << 30, outputs=True)
+        assert r[0]["steps_run"] == k
+        return o[0][0]
+    assert np.array_equal(fixed(n), w)
+    if n > 10:
+        assert np.max(np.abs(fixed(n - 10) - fixed(n - 11))) >= 1e-8
+
 
 @pytest.mark.parametrize("shape", [None, 64, -1])
 @pytest.mark.parametrize("kind", ["jacobi2", "life2"])
