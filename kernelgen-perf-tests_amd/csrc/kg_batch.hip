@@ -153,10 +153,10 @@ constexpr int var_id(int kind, int dt, int li) { return (kind * NTYPE + dt) * NS
 #define KG_BATCH_COLZ 4
 #endif
 #ifndef KG_BATCH_COLD64
-#define KG_BATCH_COLD64 2
+#define KG_BATCH_COLD64 1
 #endif
 #ifndef KG_BATCH_COLD32
-#define KG_BATCH_COLD32 4
+#define KG_BATCH_COLD32 2
 #endif
 constexpr int COLZ = KG_BATCH_COLZ;
 template <int K, int DT>

@@ -735,11 +735,15 @@ template <class C, int P = 0>
 struct CView {
     using F = typename C::F;
     const Slot<F, C::VEC, C::NH>* ring;   // [NP]
+    const F (*sh)[2 * C::H];              // [NP]: the H points left and right of the piece
     bool wr;                              // the lane's points lie in the (padded) row
     // (x + v + dx, j + dy) of the lane's plane; |dx|, |dy| <= H
     __device__ __forceinline__ F at(int v, int dx, int dy, int) const
     {
-        return slot_xat<C>(ring[(P + C::ZLO + dy) % C::NP], v, dx);
+        const int s = (P + C::ZLO + dy) % C::NP, e = v + dx;
+        if (e < 0) return sh[s][C::H + e];
+        if (e >= C::VEC) return sh[s][C::H + e - C::VEC];
+        return ring[s].v[e];
     }
     __device__ __forceinline__ F col(int v, int) const { return ring[(P + C::ZLO) % C::NP].v[v]; }
     __device__ __forceinline__ F side(int, int) const { return F(0); }
@@ -799,10 +803,26 @@ __device__ __forceinline__ void colmarch_tile(int nx, int ny, int ns, long long 
     bool xmask[C::VEC];
 #pragma unroll
     for (int v = 0; v < C::VEC; v++) xmask[v] = x + v >= LO && x + v < nx - HI;
+    // The points beside the piece of each row in the window (rows j-H ..
+    // j+H), taken from the neighbouring lanes once per row, when the row
+    // enters the window, and kept with the row's ring slot.
+    F sh[C::NP][2 * C::H];
+    auto shift_in = [&](const Slot<F, C::VEC, C::NH>& r, F (&out)[2 * C::H]) {
+#pragma unroll
+        for (int i = 0; i < C::H; i++) {
+            out[i] = dpp_shift<KG_DPP_WAVE_SHR1>(r.h[i], r.v[C::VEC - C::H + i]);
+            out[C::H + i] = dpp_shift<KG_DPP_WAVE_SHL1>(r.h[i], r.v[i]);
+        }
+    };
     march<C, false>(y0, y1, load, no_aux, no_stage,
                     [&](auto phase, int j, const auto& ring, const auto&) {
         constexpr int P = decltype(phase)::value;
-        CView<C, P> vw{ring, x < sy};
+        if (j == y0) {   // the first row: the rows below and above it enter together
+#pragma unroll
+            for (int r = 0; r < C::ZLO + C::ZR; r++) shift_in(ring[(P + r) % C::NP], sh[(P + r) % C::NP]);
+        }
+        shift_in(ring[(P + C::ZLO + C::ZR) % C::NP], sh[(P + C::ZLO + C::ZR) % C::NP]);
+        CView<C, P> vw{ring, (const F(*)[2 * C::H])sh, x < sy};
         op.run(vw, (long long)k * sz + (long long)j * sy, x, xmask);
     });
 }

@@ -34,7 +34,7 @@
 #define KG_GB3_W64 4
 #endif
 #ifndef KG_GB3_D64
-#define KG_GB3_D64 2
+#define KG_GB3_D64 1
 #endif
 #ifndef KG_GB3_YC64
 #define KG_GB3_YC64 16
@@ -43,7 +43,7 @@
 #define KG_GB3_W32 4
 #endif
 #ifndef KG_GB3_D32
-#define KG_GB3_D32 4
+#define KG_GB3_D32 2
 #endif
 #ifndef KG_GB3_YC32
 #define KG_GB3_YC32 16
@@ -123,7 +123,10 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
 // fields), else the plane engine.  gaussblur3 at 512^3, levels rotated,
 // first ten launches: FP32 W=4/D=4/YC=16 214 us against 255-264 on the
 // direct row body (YC=8 214, 12 225, 32 238, 64 256: fewer workgroups leave
-// a tail); FP64 W=4/D=2/YC=16 418 us against 451-472.
+// a tail); FP64 W=4/D=2/YC=16 418 us against 451-472.  Keeping each row's
+// shifted neighbour points with its ring slot (one set of DPP moves per row
+// instead of one per row and use) then gave FP32 207-209 us at D=2 (D=3
+// 214, D=4 210) and FP64 404-412 us (D=1 404, D=2 408-412).
 #ifndef KG_GB3_ENGINE
 #define KG_GB3_ENGINE 1
 #endif
