@@ -126,7 +126,9 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
 // a tail); FP64 W=4/D=2/YC=16 418 us against 451-472.  Keeping each row's
 // shifted neighbour points with its ring slot (one set of DPP moves per row
 // instead of one per row and use) then gave FP32 207-209 us at D=2 (D=3
-// 214, D=4 210) and FP64 404-412 us (D=1 404, D=2 408-412).
+// 214, D=4 210) and FP64 404-412 us (D=1 404, D=2 408-412).  A resident grid
+// looping over the tiles (no tail round) was slower at 16-64 rows per wave:
+// FP32 220-251 us, FP64 431-447 us.
 #ifndef KG_GB3_ENGINE
 #define KG_GB3_ENGINE 1
 #endif
