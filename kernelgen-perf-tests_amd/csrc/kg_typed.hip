@@ -128,7 +128,9 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
 // instead of one per row and use) then gave FP32 207-209 us at D=2 (D=3
 // 214, D=4 210) and FP64 404-412 us (D=1 404, D=2 408-412).  A resident grid
 // looping over the tiles (no tail round) was slower at 16-64 rows per wave:
-// FP32 220-251 us, FP64 431-447 us.
+// FP32 220-251 us, FP64 431-447 us; so was marching odd row chunks
+// downwards, which makes two neighbouring chunks read their 4 shared rows at
+// the same time (FP32 212-213 us, FP64 unchanged).
 #ifndef KG_GB3_ENGINE
 #define KG_GB3_ENGINE 1
 #endif
