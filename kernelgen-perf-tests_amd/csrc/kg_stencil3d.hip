@@ -405,6 +405,9 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32, kgp::F
 // orders in chunks of 2 or 4 tile rows alike, per-XCD regions 145-177 us.
 // 32-byte lanes (4 points, two 16-byte accesses, half the DPP moves per
 // point; 12 waves, 155 VGPRs: 16 waves spill) 122-128 us, bit-identical.
+// The DPP moves cost little: a timing copy with every x-neighbour taken from
+// the lane's own registers instead (wrong values, same loads and stores) ran
+// at 102 us against 103.
 #ifndef KG_GSRB_W
 #define KG_GSRB_W 16
 #endif
