@@ -405,9 +405,13 @@ template <int V> using WaveCfg = kgp::Cfg<V, 12, 2, 0, 0, 2, 2, 3, 1, 32, kgp::F
 // orders in chunks of 2 or 4 tile rows alike, per-XCD regions 145-177 us.
 // 32-byte lanes (4 points, two 16-byte accesses, half the DPP moves per
 // point; 12 waves, 155 VGPRs: 16 waves spill) 122-128 us, bit-identical.
-// The DPP moves cost little: a timing copy with every x-neighbour taken from
-// the lane's own registers instead (wrong values, same loads and stores) ran
-// at 102 us against 103.
+// The arithmetic costs little: timing copies (wrong values, same loads, LDS
+// staging, barriers and stores) ran at 102 us with every x-neighbour taken
+// from the lane's own registers instead of by DPP, 102-103 us with the
+// y-neighbours from registers instead of LDS, and 98-99 us storing the
+// centre value with no arithmetic at all, against 103-104 us.  The tile's
+// data movement sets the time: it reads 1.67x its input (16 rows loaded per
+// 12 computed, 2 x-halo lines per 8 of a 128-point row).
 #ifndef KG_GSRB_W
 #define KG_GSRB_W 16
 #endif
