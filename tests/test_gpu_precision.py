@@ -132,6 +132,37 @@ def test_gaussblur3_column_march_pieces_and_chunks(gpu_ctx, oracle, dtype, dims)
 
 
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("laplacian", "bf16"),
+                                         ("wave13pt", "f32"), ("gaussblur3", "f32"),
+                                         ("gaussblur3", "f64")])
+def test_typed_unaligned_fields(gpu_ctx, oracle, which, dtype):
+    """Fields one element past a 16-byte boundary (nx a multiple of every
+    vector width): the entry points take their one-point-per-lane paths and
+    still match the oracle bit for bit over two steps."""
+    dims = (264, 21, 9)
+    fn = f"{which}_{dtype}"
+    ins, out, coefs, rot = _case(which, dtype, dims, 41)
+    want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 2, rot, f64 if dtype == "f64" else f32)
+    host = ins + [out]
+    big = [gpu_ctx.empty((a.size + 1,), a.dtype) for a in host]
+    try:
+        for b, a in zip(big, host):
+            b.upload(np.concatenate([np.zeros(1, a.dtype), a]))
+        ptrs = [C.c_void_p(b.ptr + a.dtype.itemsize) for b, a in zip(big, host)]
+        for _ in range(2):
+            gpu_ctx.call(fn, *dims, *coefs, *ptrs)
+            ptrs = rot(ptrs)
+        gpu_ctx.sync()
+        got = [b.download()[1:] for b in big]
+    finally:
+        for b in big:
+            b.free()
+    # want[k] is the buffer in position k after two rotations
+    order = rot(rot(list(range(len(host)))))
+    for i, w in zip(order, want):
+        assert np.array_equal(got[i].view(np.uint8), w.view(np.uint8)), (fn, i)
+
+
+@pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("laplacian", "bf16"),
                                          ("wave13pt", "f32"), ("wave13pt", "bf16")])
 def test_typed_z_chunk_seams(gpu_ctx, oracle, which, dtype):
     """264x20x150: more z-planes than one chunk of any typed tile shape (the
