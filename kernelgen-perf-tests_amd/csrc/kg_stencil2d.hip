@@ -144,6 +144,9 @@ struct WhisperingOp {
 // 16-byte lanes, W=4/D=2/YC=8 589 us (8-byte lanes 610-643; later, first ten
 // launches of each: YC=4 584 us, 6 616, 12 644; D=1 633, D=3 584; hardware
 // and chunked tile orders 606 and 586, against 585-588 for this shape).
+// Whispering takes 258 registers, one wave per SIMD; held to 256 (two waves
+// per SIMD, 3 values spilled) it ran slower: 614-618 us against 572-573
+// (YC=6 615-628, 12 631-634, 16 660; D=1 605-622).
 //                     VEC W      H  NF LL LR ZL ZR D  NAUX YC
 template <int V> using J9Cfg = kgr::Cfg<V, 4, 1, 1, 1, 1, 0, 0, 2, 0, 64>;
 template <int V> using G25Cfg = kgr::Cfg<V, 4, 2, 1, 2, 2, 0, 0, 2, 0, 32>;
