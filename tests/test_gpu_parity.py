@@ -99,6 +99,38 @@ def test_gpu_tile_order_tails(gpu_ctx, oracle, name, dims):
         assert_bit_equal(got[role], want[role], f"{name} {dims} {role}")
 
 
+# Extents at and around the kernels' tile edges: 64- and 128-point row
+# pieces (3D), 256- and 512-point pieces (2D), tiles of 4, 8, 10, 12 and 16
+# rows, 8-, 32- and 64-plane z-chunks and 8-, 32- and 64-row y-chunks.
+EDGE_X3 = [127, 128, 129, 130, 255, 257, 258, 386]
+EDGE_Y3 = [9, 10, 11, 12, 13, 14, 18, 19, 22, 23, 26]
+EDGE_Z3 = [7, 8, 9, 10, 11, 33, 34, 35, 66]
+EDGE_X2 = [127, 128, 129, 255, 256, 257, 511, 512, 513, 514, 1026]
+EDGE_Y2 = [7, 8, 9, 10, 33, 34, 35, 63, 64, 65, 66, 130]
+
+
+@pytest.mark.parametrize("name", K.STENCILS)
+def test_gpu_matches_oracle_tile_edges(gpu_ctx, oracle, name):
+    """Six seeded shapes per stencil drawn from the tile-edge extents above:
+    partial and whole last tiles, chunks and row pieces; two steps, every
+    output bit-identical to the oracle."""
+    spec = K.SPECS[name]
+    rng = np.random.default_rng(K.STENCILS.index(name) + 900)
+    gpu = K.GpuBackend(gpu_ctx)
+    for si in range(6):
+        if spec.dim == 3:
+            dims = (int(rng.choice(EDGE_X3)), int(rng.choice(EDGE_Y3)), int(rng.choice(EDGE_Z3)))
+        else:
+            dims = (int(rng.choice(EDGE_X2)), int(rng.choice(EDGE_Y2)))
+        coefs, arrays = K.random_case(spec, dims, 1000 + si)
+        if name == "uxx1":
+            arrays["d1"] = np.abs(arrays["d1"]) + 1.0
+        want = oracle.run(spec, dims, coefs, arrays, 2)
+        got = gpu.run(spec, dims, coefs, arrays, 2)
+        for role in want:
+            assert_bit_equal(got[role], want[role], f"{name} {dims} {role}")
+
+
 @pytest.mark.parametrize("dims", [(512, 520, 40), (512, 256, 200)])
 def test_gpu_lapgsrb_long_z_chunks(gpu_ctx, oracle, dims):
     """lapgsrb lengthens its z-chunks when the launch would exceed about two
