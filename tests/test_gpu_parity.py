@@ -131,6 +131,20 @@ def test_gpu_matches_oracle_tile_edges(gpu_ctx, oracle, name):
             assert_bit_equal(got[role], want[role], f"{name} {dims} {role}")
 
 
+@pytest.mark.parametrize("name,dims", [("laplacian", (512, 512, 1100)), ("jacobi", (512, 600000))])
+def test_gpu_fields_above_2GiB(gpu_ctx, oracle, name, dims):
+    """Fields larger than 2^31 bytes (2.3 GB and 2.5 GB per array), so byte
+    offsets pass 2^31: one step bit-identical to the oracle.  Only a 3D
+    grid's xy-plane is limited to 2^31 bytes."""
+    spec = K.SPECS[name]
+    coefs, arrays = K.random_case(spec, dims, 4242)
+    assert all(a.nbytes > 2**31 for a in arrays.values())
+    want = oracle.run(spec, dims, coefs, arrays, 1)
+    got = K.GpuBackend(gpu_ctx).run(spec, dims, coefs, arrays, 1)
+    for role in want:
+        assert_bit_equal(got[role], want[role], f"{name} {dims} {role}")
+
+
 @pytest.mark.parametrize("dims", [(512, 520, 40), (512, 256, 200)])
 def test_gpu_lapgsrb_long_z_chunks(gpu_ctx, oracle, dims):
     """lapgsrb lengthens its z-chunks when the launch would exceed about two
