@@ -130,7 +130,10 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
 // looping over the tiles (no tail round) was slower at 16-64 rows per wave:
 // FP32 220-251 us, FP64 431-447 us; so was marching odd row chunks
 // downwards, which makes two neighbouring chunks read their 4 shared rows at
-// the same time (FP32 212-213 us, FP64 unchanged).
+// the same time (FP32 212-213 us, FP64 unchanged), and FP32 32-byte lanes
+// (two 16-byte loads; a wave's piece spans a whole 512-point row, so no
+// x-halo lines are read twice): 265-292 us over D=1-3, 8 rows per wave and
+// 8 waves per workgroup.
 #ifndef KG_GB3_ENGINE
 #define KG_GB3_ENGINE 1
 #endif
