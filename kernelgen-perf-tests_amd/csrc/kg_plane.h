@@ -828,12 +828,13 @@ __device__ __forceinline__ void colmarch_tile(int nx, int ny, int ns, long long 
 }
 
 // Single-grid launch of the column march over every plane of a dense grid:
-// tiles (bx, by, k), x fastest, then y, then plane, in hardware order.
-template <class C, int LO, int HI, class Op>
+// tiles (bx, by, k), x fastest, then y, then plane, in tile order ORD
+// (kg_tile_order).
+template <class C, int LO, int HI, class Op, int ORD = 0>
 __global__ __launch_bounds__(64 * C::W)
 void colmarch_kernel(int nx, int ny, int ns, int ntx, int nty, int kc, Op op)
 {
-    const unsigned b = blockIdx.x;
+    const unsigned b = kg_tile_order<ORD>(blockIdx.x, gridDim.x, (unsigned)ntx);
     const int bx = (int)(b % (unsigned)ntx);
     const int by = (int)((b / (unsigned)ntx) % (unsigned)nty);
     const int k = (int)(b / ((unsigned)ntx * (unsigned)nty));

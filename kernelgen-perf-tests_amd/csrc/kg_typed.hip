@@ -133,11 +133,23 @@ static int launch_rows2(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
 // the same time (FP32 212-213 us, FP64 unchanged), and FP32 32-byte lanes
 // (two 16-byte loads; a wave's piece spans a whole 512-point row, so no
 // x-halo lines are read twice): 265-292 us over D=1-3, 8 rows per wave and
-// 8 waves per workgroup.
+// 8 waves per workgroup.  Tile order (ORD, kg_tile_order), cold caches:
+// chunks of 4 tile rows per XCD 202.6-202.7 us (FP32) and 393-397 us (FP64)
+// against 206.9-207.4 and 404-407 in hardware order; chunks of 2 or 8 rows
+// alike, 16 rows FP64 412-417 us, XCD-contiguous eighths FP32 223-225 us.
+// With two levels ping-ponged (the input just written by the previous
+// launch) the chunked order keeps its gain in FP64 (390 against 405 us) but
+// loses in FP32 (227-250 against 213 us), so FP32 keeps the hardware order.
+#ifndef KG_ORD_CM64
+#define KG_ORD_CM64 4
+#endif
+#ifndef KG_ORD_CM32
+#define KG_ORD_CM32 0
+#endif
 #ifndef KG_GB3_ENGINE
 #define KG_GB3_ENGINE 1
 #endif
-template <template <class> class OpT, class T, int VEC, int W, int D, int YC, class Fill>
+template <template <class> class OpT, class T, int VEC, int W, int D, int YC, int ORD, class Fill>
 static int launch_colmarch(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
     if (nx % VEC != 0 || !al) return launch_typed<OpT, T, VEC, 8, 2, 16>(ctx, name, nx, ny, ns, al, fill);
@@ -154,7 +166,7 @@ static int launch_colmarch(kg_ctx* ctx, const char* name, int nx, int ny, int ns
     const int nxi = nx - LO - HI, nyi = ny - LO - HI;
     const int ntx = (nx + kgp::ColMarch<C>::TXW - 1) / kgp::ColMarch<C>::TXW;
     const int nty = nyi > 0 ? (nyi + W * YC - 1) / (W * YC) : 0;
-    auto kern = kgp::colmarch_kernel<C, LO, HI, Op>;
+    auto kern = kgp::colmarch_kernel<C, LO, HI, Op, ORD>;
     kg_register_kernel(name, (const void*)kern);
     if (int rc = kg_launch_begin(ctx, name)) return rc;
     if (nxi > 0 && nyi > 0 && ns > 0)
@@ -434,7 +446,7 @@ extern "C" int kg_gaussblur3_f64(kg_ctx* ctx, int nx, int ny, int ns, double s0,
     KG_REQUIRE(w0 != w1, "kg_gaussblur3_f64: w0 and w1 must be distinct");
     const double f = 1. / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);   // gaussblur/gaussblur.c:65
 #if KG_GB3_ENGINE == 1
-    return launch_colmarch<Gaussblur3Op, kgp::F64, 2, KG_GB3_W64, KG_GB3_D64, KG_GB3_YC64>(ctx, "gaussblur3_f64", nx, ny, ns,
+    return launch_colmarch<Gaussblur3Op, kgp::F64, 2, KG_GB3_W64, KG_GB3_D64, KG_GB3_YC64, KG_ORD_CM64>(ctx, "gaussblur3_f64", nx, ny, ns,
 #else
     return launch_rows2<Gaussblur3Op, kgp::F64, 2, 8, 4>(ctx, "gaussblur3_f64", nx, ny, ns,
 #endif
@@ -458,7 +470,7 @@ extern "C" int kg_gaussblur3_f32(kg_ctx* ctx, int nx, int ny, int ns, float s0, 
     KG_REQUIRE(w0 != w1, "kg_gaussblur3_f32: w0 and w1 must be distinct");
     const float f = 1.f / (s0 + 4 * (s1 + s2 + s4 + s8) + 8 * s5);
 #if KG_GB3_ENGINE == 1
-    return launch_colmarch<Gaussblur3Op, kgp::F32, 4, KG_GB3_W32, KG_GB3_D32, KG_GB3_YC32>(ctx, "gaussblur3_f32", nx, ny, ns,
+    return launch_colmarch<Gaussblur3Op, kgp::F32, 4, KG_GB3_W32, KG_GB3_D32, KG_GB3_YC32, KG_ORD_CM32>(ctx, "gaussblur3_f32", nx, ny, ns,
 #else
     return launch_rows2<Gaussblur3Op, kgp::F32, 4, 8, 8>(ctx, "gaussblur3_f32", nx, ny, ns,
 #endif
