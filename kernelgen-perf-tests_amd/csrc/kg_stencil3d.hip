@@ -83,7 +83,10 @@ void divergence_k(int nx, int ny, int ns, int ntx, int nty, double alpha, double
 // j-1, j+1; uz at planes k0-1 .. k0+PZ), all issued before the first use.
 // Workgroups stay in hardware order, at most DIV_OCC (16 waves) per compute
 // unit: 183 us at 512x256x256 (unlimited 187, 3 per CU 196; uxx1 gains
-// nothing from such a cap).  Same expression as divergence_k.
+// nothing from such a cap).  Other tile orders (kg_tile_order) measured no
+// better: chunks of 2, 4 or 8 tile rows per XCD 183-186 us (uxx1 286-295
+// against 284-286 in hardware order), XCD-contiguous eighths 188 us (uxx1
+// 310).  Same expression as divergence_k.
 #ifndef KG_DIV_PZ
 #define KG_DIV_PZ 4
 #endif
