@@ -30,6 +30,20 @@
 #ifndef KG_ORD_TYPED
 #define KG_ORD_TYPED 2
 #endif
+// BF16 wave13pt (64-plane z-chunks sharing 4 planes at each seam) enumerates
+// its tiles z-chunk fastest and deals them to the XCDs in runs of 2 whole
+// z-columns (plane_kernel's ORD < 0), so the chunks of a column run on one
+// XCD: 170-177 us on the first ten launches at 512^3 (179-181 mean, three
+// levels rotated) against 189-196 (193) in chunks of 2 tile rows.  FP32
+// wave13pt lost that way (runs of 1, 2, 4 columns: 326-351 us against 309-311)
+// and so did both laplacians (FP32 232-248 us against 201-203; BF16 158-168
+// against 137-138, with 8-plane chunks).
+#ifndef KG_ORD_WF32
+#define KG_ORD_WF32 KG_ORD_TYPED
+#endif
+#ifndef KG_ORD_WBF
+#define KG_ORD_WBF -2
+#endif
 #ifndef KG_GB3_W64
 #define KG_GB3_W64 4
 #endif
@@ -61,7 +75,8 @@ using namespace kgops;
 
 // Single-grid launch of op Op in storage type T: 16-byte lanes (VEC = VMAX)
 // when nx is a multiple of VMAX, else one point per lane.
-template <template <class> class OpT, class T, int VMAX, int W, int D, int KC, class Fill>
+template <template <class> class OpT, class T, int VMAX, int W, int D, int KC, int ORD = KG_ORD_TYPED,
+          class Fill>
 static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, bool al, Fill fill)
 {
     if (int rc = kg_check_ctx(ctx)) return rc;
@@ -76,7 +91,7 @@ static int launch_typed(kg_ctx* ctx, const char* name, int nx, int ny, int ns, b
         Op op{};
         fill(op);
         const kgp::Grid g = kgp::make_grid<C>(nx, ny, ns, LO, HI, ZLO, ZHI, KC);
-        auto kern = kgp::plane_kernel<C, LO, HI, Op, ZLO, ZHI, KG_ORD_TYPED>;
+        auto kern = kgp::plane_kernel<C, LO, HI, Op, ZLO, ZHI, ORD>;
         kg_register_kernel(name, (const void*)kern);
         if (int rc = kg_launch_begin(ctx, name)) return rc;
         if (!g.empty)
@@ -407,7 +422,7 @@ extern "C" int kg_wave13pt_f32(kg_ctx* ctx, int nx, int ny, int ns, float m0, fl
 {
     KG_REQUIRE(w0 && w1 && w2, "kg_wave13pt_f32: null buffer");
     KG_REQUIRE(w2 != w0 && w2 != w1, "kg_wave13pt_f32: w2 aliases an input");
-    return launch_typed<Wave13ptOp, kgp::F32, 4, KG_WF32_W, KG_WF32_D, KG_WF32_KC>(ctx, "wave13pt_f32", nx, ny, ns,
+    return launch_typed<Wave13ptOp, kgp::F32, 4, KG_WF32_W, KG_WF32_D, KG_WF32_KC, KG_ORD_WF32>(ctx, "wave13pt_f32", nx, ny, ns,
                                                             kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
@@ -427,7 +442,7 @@ extern "C" int kg_wave13pt_bf16(kg_ctx* ctx, int nx, int ny, int ns, float m0, f
     // 168 registers a lane: the 16-wave tile would spill at 128), 64 planes
     // per workgroup, 2-plane prefetch; the measurements are listed with the
     // KG_WBF_* defaults above.
-    return launch_typed<Wave13ptOp, kgp::BF16, KG_WBF_V, KG_WBF_W, KG_WBF_D, KG_WBF_KC>(ctx, "wave13pt_bf16", nx, ny, ns,
+    return launch_typed<Wave13ptOp, kgp::BF16, KG_WBF_V, KG_WBF_W, KG_WBF_D, KG_WBF_KC, KG_ORD_WBF>(ctx, "wave13pt_bf16", nx, ny, ns,
                                                              kg_aligned16(w0, w1, w2), [&](auto& op) {
         op.src = w1;
         op.side[0] = w0;
