@@ -19,7 +19,11 @@
 #include <type_traits>
 
 // Tile order per kernel (kg_tile_order: 0 hardware, 1 XCD-contiguous, R >= 2
-// chunks of R tile rows per XCD), measured at 512x256x256.
+// chunks of R tile rows per XCD; plane_kernel's -R: z-chunk fastest, runs of
+// R whole z-columns per XCD), measured at 512x256x256.  wave13pt in chunks of
+// 2 tile rows 141.5-142 us on the first ten launches against 144.7-145.3
+// XCD-contiguous (z-column runs of 1, 2, 4: 148-153); the z-column runs were
+// slower for laplacian too (111-121 us against 98-99) and alike for lapgsrb.
 #ifndef KG_ORD_LAP
 #define KG_ORD_LAP 2
 #endif
@@ -27,7 +31,7 @@
 #define KG_ORD_GRAD 2
 #endif
 #ifndef KG_ORD_WAVE
-#define KG_ORD_WAVE 1
+#define KG_ORD_WAVE 2
 #endif
 #ifndef KG_ORD_GSRB
 #define KG_ORD_GSRB 0
