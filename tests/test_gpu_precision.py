@@ -89,9 +89,10 @@ def test_typed_matches_oracle(gpu_ctx, oracle, which, dtype):
 @pytest.mark.parametrize("which,dtype", [("laplacian", "f32"), ("wave13pt", "f32"),
                                          ("laplacian", "bf16"), ("wave13pt", "bf16")])
 def test_typed_tile_order_tail(gpu_ctx, oracle, which, dtype):
-    """600x300x40: 450 (FP32 laplacian), 500 (BF16 laplacian) and 75
-    (wave13pt) workgroups, not whole rounds of 8 chunks of two tile rows; the
-    last ones keep their hardware ids."""
+    """600x300x40: 450 (FP32 laplacian), 500 (BF16 laplacian) and 75 / 74
+    (FP32 / BF16 wave13pt) workgroups, not whole rounds of 8 chunks (two tile
+    rows, or for BF16 wave13pt two whole z-columns); the last ones keep their
+    hardware ids."""
     dims = (600, 300, 40)
     fn = f"{which}_{dtype}"
     ins, out, coefs, rot = _case(which, dtype, dims, 9)
