@@ -37,7 +37,8 @@
 // levels rotated) against 189-196 (193) in chunks of 2 tile rows.  FP32
 // wave13pt lost that way (runs of 1, 2, 4 columns: 326-351 us against 309-311)
 // and so did both laplacians (FP32 232-248 us against 201-203; BF16 158-168
-// against 137-138, with 8-plane chunks).
+// against 137-138 with 8-plane chunks, 160-173 with 16-64-plane chunks in
+// z-column runs or chunked tile rows).
 #ifndef KG_ORD_WF32
 #define KG_ORD_WF32 KG_ORD_TYPED
 #endif
