@@ -29,10 +29,10 @@
 // Measured on MI355X at 512x256x256 FP64 (cold caches): laplacian 99-102 us
 // with this engine against 143 us for per-thread z-marching with
 // cache-served neighbours.  HBM traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE
-// against the algorithmic bytes) is 1.03x for wave13pt but 1.18x for
-// laplacian and 1.22x for lapgsrb: tiles 512 bytes wide re-fetch the x-halo
-// lines their neighbours on other XCDs also read, and the y-halo rows of a
-// tile are read again by the tiles above and below it.
+// against the algorithmic bytes) is 1.03-1.09x for wave13pt (by tile
+// order) but 1.17x for laplacian and 1.29x for lapgsrb: tiles 512 bytes wide
+// re-fetch the x-halo lines their neighbours on other XCDs also read, and
+// the y-halo rows of a tile are read again by the tiles above and below it.
 //
 // The header also holds the engines for stencils that read only their own
 // xy-plane: packed rows (narrow grids), the direct row body (rows loaded
