@@ -89,7 +89,7 @@ def test_gpu_tile_order_tails(gpu_ctx, oracle, name, dims):
     """Grids whose workgroup count is not a whole number of tile-order rounds
     (8 chunks of tile rows, or 8 XCD-contiguous parts): the workgroups past
     the last whole round keep their hardware ids.  Launches here: laplacian
-    300 workgroups (rounds of 48), gradient 300 (48), wave13pt 150 (8),
+    300 workgroups (rounds of 48), gradient 300 (48), wave13pt 150 (48),
     jacobi and gaussblur 175 (160), gameoflife 345 (160)."""
     spec = K.SPECS[name]
     coefs, arrays = K.random_case(spec, dims, 88)
