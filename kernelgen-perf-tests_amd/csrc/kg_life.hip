@@ -128,7 +128,9 @@ void life_u8_cells_k(int nx, int ny, int ntx, const uint8_t* __restrict__ c0,
 // slower).  Tile order KG_ORD_LIFE8: chunks of 8 bands per XCD, so a band's
 // neighbour rows and segments share its L2 (ping-pong steps, 50-launch means:
 // 8192^2 30.7 us, 16384^2 100 us; hardware order 31.4 / 107, XCD-contiguous
-// 37.9 / 108, chunks of 2, 4 or 16 bands within 2 %).
+// 37.9 / 108, chunks of 2, 4 or 16 bands within 2 %).  At 8192^2, 2 or 8
+// segments per workgroup ran 33.5 and 32.2 us against 31.0 for 4; 2 rows in
+// flight the same, 5 rows 32.4 us; two ring turns per band (D = 1) 31.1 us.
 __global__ __launch_bounds__(64 * LIFE_WAVES)
 void life_u8_k(int nx, int ny, int nsx, const uint8_t* __restrict__ c0, uint8_t* __restrict__ c1)
 {
