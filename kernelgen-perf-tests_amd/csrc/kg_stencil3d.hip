@@ -351,7 +351,9 @@ struct LapgsrbOp {
 // chunks of 2 tile rows per XCD, W=12/D=2/KC=8 100 us (W=8 102, W=16 102,
 // D=3 102, KC=16 105, KC=4 107; XCD-contiguous order W=8 106; two or four
 // rows per wave, 64- and 32-point tiles with fewer y-halo rows, 101-107;
-// z-chunks lengthened to 2-8 workgroups per CU, 14-64 planes, 102-108);
+// z-chunks lengthened to 2-8 workgroups per CU, 14-64 planes, 102-108;
+// 32-byte lanes, 256-point tiles with half the x-halo lines, 113-128 over
+// W=8-16, D=2-3, KC=8-16);
 // wave13pt
 // W=12/D=3 141 us (W=16 148, W=14 169; z-chunks lengthened to 2 workgroups
 // per CU 172); lapgsrb with 8-byte lanes, W=12/D=2, hardware order, z-chunks
