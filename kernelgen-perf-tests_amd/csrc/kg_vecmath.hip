@@ -133,7 +133,10 @@ void matvec_k(int nx, int ny, const T* __restrict__ A, const T* __restrict__ x, 
 // the lane count, decide the speed.  The sum is the reference's: i
 // ascending, product then add.  FP64 chunks of 16, 48 or 64 values (instead
 // of 32) measured the same 61-64 us at 512 x 65536: with every lane on its
-// own 4 KB row, the reads are scattered over the whole matrix at once.
+// own 4 KB row, the reads are scattered over the whole matrix at once.  A
+// row-block form that reads 8 whole rows per workgroup as one contiguous
+// block into LDS and sums them there (8 lanes per workgroup) was
+// bit-identical and slower, 115 us: the serial sums then dominate.
 template <class T> constexpr int mv_chunk() { return sizeof(T) == 8 ? 32 : 64; }
 
 template <class T>
