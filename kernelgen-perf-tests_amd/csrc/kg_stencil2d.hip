@@ -183,7 +183,10 @@ static int launch_row(kg_ctx* ctx, const char* name, int nx, int ny, bool al, Fi
 // sweeps at 512x65536 (cold caches, against the row engine's 113-117 us),
 // in chunked tile order: jacobi PY=8/4 waves 102 us (PY=4: 103, 8 waves
 // 104), gameoflife PY=4/4 waves 101 us (PY=8: 107); gaussblur PY=8/4 waves
-// 95 us (PY=4/8 waves 111, PY=8/8 waves 106).
+// 95 us (PY=4/8 waves 111, PY=8/8 waves 106).  Rechecked on the final
+// kernels: jacobi PY=12 112 us, PY=16 105, 2 or 8 waves 101-103 against
+// 100; gameoflife PY=8 104, 2 or 8 waves 102-104 against 99; gaussblur
+// PY=12 99-100 against 94.
 #ifndef KG_J9_PY
 #define KG_J9_PY 4
 #endif
