@@ -116,14 +116,15 @@ def test_typed_full_size_one_step(gpu_ctx, oracle, which, dtype):
         assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
 
 
-@pytest.mark.parametrize("dims", [(264, 150, 3), (520, 71, 2), (128, 200, 1)])
+@pytest.mark.parametrize("dims", [(264, 150, 3), (520, 71, 2), (128, 200, 1), (256, 300, 40)])
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_gaussblur3_column_march_pieces_and_chunks(gpu_ctx, oracle, dtype, dims):
     """Per-plane gaussblur on the column march: rows of several pieces (the
     x-neighbours across a piece boundary come from the halo points the end
     lanes load, and a last piece mostly past the row), row chunks of 16 rows
-    per wave with a partial last workgroup, one to three planes; three steps,
-    bit for bit against the oracle."""
+    per wave with a partial last workgroup, one to forty planes (256x300x40
+    in FP64: 400 workgroups, 6 whole rounds of the XCD-chunked tile order and
+    a tail); three steps, bit for bit against the oracle."""
     fn = f"gaussblur3_{dtype}"
     ins, out, coefs, rot = _case("gaussblur3", dtype, dims, 31)
     want = _run_oracle(oracle.lib, fn, dims, coefs, ins, out, 3, rot, f64 if dtype == "f64" else f32)
@@ -176,6 +177,18 @@ def test_typed_z_chunk_seams(gpu_ctx, oracle, which, dtype):
     got = _run_gpu(gpu_ctx, fn, dims, coefs, ins, out, 2, rot)
     for g, w in zip(got, want):
         assert np.array_equal(g.view(np.uint8), w.view(np.uint8)), fn
+
+
+def test_wave13pt_bf16_z_column_runs(gpu_ctx, oracle):
+    """512x200x150 BF16 wave13pt: 25 tile rows of 3 z-chunks each (75
+    workgroups), dealt to the XCDs in runs of 2 whole z-columns -- one whole
+    round of 8 runs and a tail; two steps, bit for bit."""
+    dims = (512, 200, 150)
+    ins, out, coefs, rot = _case("wave13pt", "bf16", dims, 29)
+    want = _run_oracle(oracle.lib, "wave13pt_bf16", dims, coefs, ins, out, 2, rot, f32)
+    got = _run_gpu(gpu_ctx, "wave13pt_bf16", dims, coefs, ins, out, 2, rot)
+    for g, w in zip(got, want):
+        assert np.array_equal(g.view(np.uint8), w.view(np.uint8))
 
 
 def _oracle_to_bf16(oracle):
