@@ -112,7 +112,10 @@ void pj_smooth4_k(int N, int P, int ntx, float c0, float c1, float c2, float cf,
 // its own points.  u rows j0-2 .. j0+PY+1 and f rows j0-1 .. j0+PY come
 // through the caches; HBM sees each array once per two sweeps.
 #ifndef KG_SM2_PY
-#define KG_SM2_PY 2   // rows per lane (8192^2 solve: 19.0 ms; 3 rows 19.1, 4 rows 19.4, 6 rows 19.0)
+// rows per lane: 8192^2 solve 19.0 ms (3 rows 19.1, 4 rows 19.4, 6 rows 19.0);
+// with the chunked tile order 17.0-17.2 ms (3 rows 17.2-17.3, 4 rows 17.5,
+// 6 rows 17.2-17.3)
+#define KG_SM2_PY 2
 #endif
 template <int PY>
 __global__ __launch_bounds__(64 * VW)
